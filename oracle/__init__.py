@@ -1,0 +1,153 @@
+"""FP64 oracle of the libconv separable convolutions -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package, and only as the checker (or as the CPU baseline that is
+timed beside the GPU path).  The product library under boast-lig_amd/ never
+loads it.
+
+`liblc_oracle.so` (built by `make -C oracle`) is the C restatement of the
+BOAST generator's semantics (BigDFT/GenericConvolution2.rb); the `ref`
+submodule wraps the reference's own FP64 Fortran oracles compiled into
+oracle/_ref/ by build_ref.py (present only where the reference checkout was
+available at build time).
+
+Arrays are numpy float64, Fortran order (dim 0 fastest), flattened the way
+the libconv entry points see them.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MD, IMD, D2, D1 = 0, 1, 2, 3
+SYM8, SYMICOMB8 = 0, 1
+PERIODIC, GROW, SHRINK = 0, 1, -1
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, 'liblc_oracle.so')
+        if not os.path.exists(path):
+            raise RuntimeError('oracle not built: run make -C oracle')
+        L = ctypes.CDLL(path)
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i, d = ctypes.c_int, ctypes.c_double
+        L.lco_s0s0_1d.argtypes = [i, i, i, ip, i, ip, ip, i, dp, dp, d, d, d, dp]
+        L.lco_s0s1_1d.argtypes = [i, i, i, ip, i, ip, ip, i, dp, dp, d, d]
+        L.lco_s1s0_1d.argtypes = [i, i, i, ip, i, ip, ip, i, dp, dp, d, d]
+        L.lco_s0s0_md.argtypes = [i, i, ip, ip, ip, ip, i, dp, dp, d, d, d]
+        L.lco_s0s0_1ds.argtypes = [i, i, ip, ip, ip, ip, i, dp, d, d, dp, dp, dp]
+        L.lco_s0s1_md.argtypes = [i, i, ip, ip, ip, ip, i, dp, dp, d, d]
+        L.lco_s1s0_md.argtypes = [i, i, ip, ip, ip, ip, i, dp, dp, d, d]
+        L.lco_filter_taps.argtypes = [i, ip, dp, i]
+        for f in ('lco_s0s0_1d', 'lco_s0s1_1d', 'lco_s1s0_1d', 'lco_s0s0_md', 'lco_s0s0_1ds',
+                  'lco_s0s1_md', 'lco_s1s0_md', 'lco_filter_taps', 'lco_num_threads'):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _iarr(v):
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.int32))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _darr(a):
+    assert a.dtype == np.float64 and (a.flags['C_CONTIGUOUS'] or a.flags['F_CONTIGUOUS'])
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ValueError('oracle %s rejected its arguments' % what)
+
+
+def num_threads():
+    return lib().lco_num_threads()
+
+
+def taps(fil):
+    out = np.zeros(64)
+    lo = ctypes.c_int()
+    nt = lib().lco_filter_taps(fil, ctypes.byref(lo), _darr(out), 64)
+    return lo.value, out[:nt].copy()
+
+
+def s0s0_1d(fil, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0, want_dot=False):
+    """Convolution sweep along idim (libconv s0s0_1d); y updated in place."""
+    (n_, pn), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(nx), _iarr(ny)
+    dot = ctypes.c_double(0.0)
+    rc = lib().lco_s0s0_1d(fil, len(n), idim, pn, bc, pnx, pny, narr, _darr(x), _darr(y),
+                           a, a_x, a_y, ctypes.byref(dot) if want_dot else None)
+    _check(rc, 's0s0_1d')
+    return dot.value if want_dot else None
+
+
+def s0s1_1d(wav, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0):
+    (n_, pn), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(nx), _iarr(ny)
+    _check(lib().lco_s0s1_1d(wav, len(n), idim, pn, bc, pnx, pny, narr, _darr(x), _darr(y), a, a_y),
+           's0s1_1d')
+
+
+def s1s0_1d(wav, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0):
+    (n_, pn), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(nx), _iarr(ny)
+    _check(lib().lco_s1s0_1d(wav, len(n), idim, pn, bc, pnx, pny, narr, _darr(x), _darr(y), a, a_y),
+           's1s0_1d')
+
+
+def s0s0_md(fil, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0):
+    (n_, pn), (b_, pb), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(bc), _iarr(nx), _iarr(ny)
+    _check(lib().lco_s0s0_md(fil, len(n), pn, pb, pnx, pny, narr, _darr(x), _darr(y), a, a_x, a_y),
+           's0s0_md')
+
+
+def s0s0_1ds(fil, n, bc, nx, ny, narr, a, a_x, a_y, x, y):
+    """Kinetic-style sum over dims; returns the per-dim dots."""
+    (n_, pn), (b_, pb), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(bc), _iarr(nx), _iarr(ny)
+    av = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    dot = np.zeros(len(n))
+    _check(lib().lco_s0s0_1ds(fil, len(n), pn, pb, pnx, pny, narr, _darr(av), a_x, a_y,
+                              _darr(x), _darr(y), _darr(dot)), 's0s0_1ds')
+    return dot
+
+
+def s0s1_md(wav, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0):
+    (n_, pn), (b_, pb), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(bc), _iarr(nx), _iarr(ny)
+    _check(lib().lco_s0s1_md(wav, len(n), pn, pb, pnx, pny, narr, _darr(x), _darr(y), a, a_y),
+           's0s1_md')
+
+
+def s1s0_md(wav, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0):
+    (n_, pn), (b_, pb), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(bc), _iarr(nx), _iarr(ny)
+    _check(lib().lco_s1s0_md(wav, len(n), pn, pb, pnx, pny, narr, _darr(x), _darr(y), a, a_y),
+           's1s0_md')
+
+
+# ---------------------------------------------------------------------------
+# size rules (BigDFT/LibconvBase.rb:98-135 outdim/get_ld; GenericConvolution2
+# .rb:1584-1618): extent along a dim of the input / output of an operator.
+FILTER_LEN = {MD: 16, IMD: 16, D2: 29, D1: 29}
+WAVELET_PAIRS = {SYM8: 8, SYMICOMB8: 16}
+
+
+def conv_extents(fil, bc, n):
+    L = FILTER_LEN[fil]
+    if bc == GROW:
+        return n, n + L - 1
+    if bc == SHRINK:
+        return n + L - 1, n
+    return n, n
+
+
+def wavelet_extents(wav, bc, n):
+    W = WAVELET_PAIRS[wav]
+    if bc == GROW:
+        return n, n + W - 1
+    if bc == SHRINK:
+        return n + W - 1, n
+    return n, n

@@ -1,0 +1,91 @@
+"""ctypes wrappers of the reference's own FP64 oracles in oracle/_ref/.
+
+TEST INFRASTRUCTURE ONLY.  The shared objects are compiled from the
+read-only reference checkout by oracle/build_ref.py; where they are absent
+(no checkout at build time) `available()` is False and callers skip.
+Fortran calling convention: every argument by reference.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), '_ref')
+_libs = {}
+
+
+def _load(name):
+    if name not in _libs:
+        p = os.path.join(HERE, name)
+        _libs[name] = ctypes.CDLL(p) if os.path.exists(p) else None
+    return _libs[name]
+
+
+def available(name='libref_mf.so'):
+    return _load(name) is not None
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _d(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags['F_CONTIGUOUS']
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def magicfilter(kind, n1, ndat, x):
+    """magicfilter_{per,free}[_t]_ref (BigDFT/magicfilter_refs.f90): x is
+    (n_in, ndat) Fortran order, returns y (ndat, n_out) -- the transposed
+    output of the reference routine."""
+    L = _load('libref_mf.so')
+    n_out = {'per': n1, 'per_t': n1, 'free': n1 + 15, 'free_t': n1}[kind]
+    y = np.zeros((ndat, n_out), order='F')
+    getattr(L, 'magicfilter_%s_ref_' % kind)(_i(n1), _i(ndat), _p(x), _p(y))
+    return y
+
+
+def kinetic_per(n, hgrid, x, c):
+    """kinetic_per_ref (BigDFT/Kinetic.rb:981-1061): y = c x - 1/2 Lap x."""
+    L = _load('libref_kin.so')
+    h = np.asfortranarray(np.asarray(hgrid, dtype=np.float64))
+    y = np.zeros(n, order='F')
+    L.kinetic_per_ref_(_i(n[0]), _i(n[1]), _i(n[2]), _p(h), _p(x), _p(y), _d(c))
+    return y
+
+
+def kinetic_ekin(n, hgrid, x, y):
+    """kinetic_per_ref_optim_ekin (BigDFT/Kinetic.rb:21-112): y += -1/2 Lap x
+    (periodic) and returns the per-direction <x, -1/2 d2/dxi2 x>."""
+    L = _load('libref_kin.so')
+    h = np.asfortranarray(np.asarray(hgrid, dtype=np.float64))
+    k = np.zeros(6, order='F')
+    L.kinetic_per_ref_optim_ekin_(_i(n[0] - 1), _i(n[1] - 1), _i(n[2] - 1), _p(h), _p(x), _p(y), _p(k))
+    return k[:3].copy()
+
+
+def idwt_1d(d, idim, n, bc, nx, ny, narr, x, y, a, a_y):
+    """BOAST-generated d_s1s0_1d_sym8 dispatcher (BigDFT/Wavelet1d-nb.ipynb
+    cell 4): y := a * IDWT_idim(x) + a_y * y."""
+    L = _load('libref_idwt.so')
+    na = np.asfortranarray(np.asarray(n, dtype=np.int32))
+    nxa = np.asfortranarray(np.asarray(nx, dtype=np.int32))
+    nya = np.asfortranarray(np.asarray(ny, dtype=np.int32))
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.d_s1s0_1d_sym8_(_i(d), _i(idim), na.ctypes.data_as(ip), _i(bc), nxa.ctypes.data_as(ip),
+                      nya.ctypes.data_as(ip), _i(narr), _p(x), _p(y), _d(a), _d(a_y))
+
+
+def bigdft_wavelet(kind, n, ndat, x):
+    """BigDFT's own periodic synthesis/analysis (BigDFT/old/Synthesis.rb:100-146,
+    BigDFT/old/AnaRotPer-2.rb:86-221) on n pairs: x (2n, ndat) -> y (ndat, 2n).
+    analysis_per_ref counts pairs from zero (its argument is n-1)."""
+    L = _load('libref_wav.so')
+    y = np.zeros((ndat, 2 * n), order='F')
+    arg = n if kind == 'synthesis_per' else n - 1
+    getattr(L, '%s_ref_' % kind)(_i(arg), _i(ndat), _p(x), _p(y))
+    return y

@@ -1,0 +1,112 @@
+#!/usr/bin/env python3
+"""Write the golden vectors of tests/golden/ from the reference's own oracles.
+
+Runs only where oracle/_ref/ was built from the reference checkout
+(oracle/build_ref.py).  Every fixture holds seeded inputs and the outputs
+that the reference's FP64 code produced for them:
+
+  mf.npz      magicfilter_{per,per_t,free,free_t}_ref   BigDFT/magicfilter_refs.f90
+  kinetic.npz kinetic_per_ref (c = 0.5, h = (0.5, 0.6, 0.7)) and the
+              per-direction energies of kinetic_per_ref_optim_ekin
+                                                        BigDFT/Kinetic.rb:21-112,981-1061
+  idwt.npz    the BOAST-generated d_s1s0_1d_sym8 (every idim x BC x a/a_y,
+              padded leading dims, narr = 2)            BigDFT/Wavelet1d-nb.ipynb cell 4
+  bigdft_wavelet.npz  BigDFT's own synthesis_per_ref / analysis_per_ref
+                                                        BigDFT/old/Synthesis.rb,
+                                                        BigDFT/old/AnaRotPer-2.rb
+
+Usage: python3 tests/golden/make_golden.py  (from the repository root).
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from oracle import ref as R  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+SYM8_PAIRS = 8
+
+
+def mf():
+    rng = np.random.default_rng(101)
+    n1, ndat = 37, 11
+    x = np.asfortranarray(rng.uniform(-1, 1, (n1, ndat)))
+    xs = np.asfortranarray(rng.uniform(-1, 1, (n1 + 15, ndat)))
+    np.savez_compressed(os.path.join(OUT, 'mf.npz'), n1=n1, ndat=ndat, x=x, x_shrink=xs,
+                        per=R.magicfilter('per', n1, ndat, x),
+                        per_t=R.magicfilter('per_t', n1, ndat, x),
+                        free=R.magicfilter('free', n1, ndat, x),
+                        free_t=R.magicfilter('free_t', n1, ndat, xs))
+
+
+def kinetic():
+    rng = np.random.default_rng(202)
+    n = (12, 14, 13)
+    h = (0.5, 0.6, 0.7)
+    x = np.asfortranarray(rng.uniform(-1, 1, n))
+    y = R.kinetic_per(n, h, x, 0.5)
+    yk = np.zeros(n, order='F')
+    kstr = R.kinetic_ekin(n, h, x, yk)
+    np.savez_compressed(os.path.join(OUT, 'kinetic.npz'), n=np.array(n), h=np.array(h), c=0.5,
+                        x=x, y=y, y_ekin=yk, kstrten=kstr)
+
+
+def idwt():
+    # the processed dim holds 9 pairs: the generated free-BC sweeps read out
+    # of bounds below the filter width (8 pairs), so the fixtures stay above it
+    rng = np.random.default_rng(303)
+    cases = {}
+    k = 0
+    for idim in range(3):
+        n = [2, 3, 2]
+        n[idim] = 9
+        for bc in (0, 1, -1):
+            for a, a_y in ((1.0, 0.0), (1.3, -0.4)):
+                nx, ny = list(n), list(n)
+                w = SYM8_PAIRS - 1
+                ein = n[idim] + (w if bc == -1 else 0)
+                eout = n[idim] + (w if bc == 1 else 0)
+                nx[idim], ny[idim] = ein + 1, eout + 1   # padded leading dims
+                narr = 2
+                nin = int(np.prod([2 * v for v in nx]))
+                nout = int(np.prod([2 * v for v in ny]))
+                x = np.asfortranarray(rng.uniform(-1, 1, nin * narr))
+                y0 = np.asfortranarray(rng.uniform(-1, 1, nout * narr))
+                y = y0.copy(order='F')
+                R.idwt_1d(3, idim, n, bc, nx, ny, narr, x, y, a, a_y)
+                cases['c%d' % k] = np.array([idim, bc, narr] + n + nx + ny, dtype=np.int32)
+                cases['s%d' % k] = np.array([a, a_y])
+                cases['x%d' % k] = x
+                cases['y0_%d' % k] = y0
+                cases['y%d' % k] = y
+                k += 1
+    np.savez_compressed(os.path.join(OUT, 'idwt.npz'), ncases=k, **cases)
+
+
+def bigdft_wavelet():
+    rng = np.random.default_rng(404)
+    n, ndat = 10, 7
+    x = np.asfortranarray(rng.uniform(-1, 1, (2 * n, ndat)))
+    np.savez_compressed(os.path.join(OUT, 'bigdft_wavelet.npz'), n=n, ndat=ndat, x=x,
+                        synthesis_per=R.bigdft_wavelet('synthesis_per', n, ndat, x),
+                        analysis_per=R.bigdft_wavelet('analysis_per', n, ndat, x))
+
+
+def main():
+    for lib in ('libref_mf.so', 'libref_kin.so', 'libref_idwt.so', 'libref_wav.so'):
+        if not R.available(lib):
+            print('make_golden: oracle/_ref/%s missing (run make -C oracle ref)' % lib)
+            return 1
+    mf()
+    kinetic()
+    idwt()
+    bigdft_wavelet()
+    print('make_golden: wrote', sorted(f for f in os.listdir(OUT) if f.endswith('.npz')))
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

@@ -1,0 +1,302 @@
+"""Pins the FP64 oracle (oracle/liblc_oracle.so) before it is trusted.
+
+Each check is chosen so that a plausible mistake in the restatement (a wrong
+tap or sign, a transposed index, an off-by-one origin, a dropped epilogue
+term) fails it:
+  * golden vectors written by the reference's own FP64 code (tests/golden/,
+    see make_golden.py) -- MagicFilter (4 variants), kinetic + per-direction
+    energies, the BOAST-generated IDWT dispatcher over every idim x BC x
+    scalar variant, BigDFT's own periodic synthesis;
+  * the live reference oracles in oracle/_ref/ when they were built here;
+  * mathematics: tap identities, the second-derivative filter as the
+    eigenvector of the refinement operator of the low-pass filter, the magic
+    filter's moments against the scaling function's, plane-wave
+    eigenvalues, adjointness, orthonormality and round trips.
+"""
+import os
+from math import comb, sqrt
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import ref as R
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def lp_taps():
+    import importlib.util
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        'boast-lig_amd', 'csrc', 'gen_taps.py')
+    spec = importlib.util.spec_from_file_location('gen_taps', path)
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return np.array([float(s) for s in m.SYM8_LP])
+
+
+# --------------------------------------------------------------------------
+# taps
+def test_tap_identities():
+    lo, md = O.taps(O.MD)
+    assert lo == -8 and len(md) == 16
+    assert abs(md.sum() - 1) < 1e-15
+    lo, imd = O.taps(O.IMD)
+    assert lo == -7 and np.array_equal(imd, md[::-1])
+    lo, d2 = O.taps(O.D2)
+    l = np.arange(lo, lo + len(d2))
+    assert lo == -14 and np.array_equal(d2, d2[::-1])
+    assert abs(d2.sum()) < 1e-14 and abs((l ** 2 * d2).sum() - 2) < 1e-13
+    assert abs(np.abs(d2).sum() - 9.915) < 1e-3
+    lo, d1 = O.taps(O.D1)
+    assert np.array_equal(d1, -d1[::-1])
+    assert abs((l * d1).sum() - 1) < 1e-14 and abs((l ** 3 * d1).sum()) < 1e-13
+    h = lp_taps()
+    assert abs(h.sum() - sqrt(2)) < 1e-15 and abs((h * h).sum() - 1) < 1e-15
+    for m in range(1, 8):
+        assert abs((h[:-2 * m] * h[2 * m:]).sum()) < 1e-15
+
+
+def test_d2_is_refinement_eigenvector():
+    """r_l = Phi''(l) for the autocorrelation Phi of the scaling function
+    satisfies r = 4 A r with A[l][k] = a_{2l-k}, a_k = sum_i h_i h_{i+k}."""
+    h = lp_taps()
+    a = np.correlate(h, h, 'full')  # a[k+15] = sum_i h_i h_{i+k}
+    lo, d2 = O.taps(O.D2)
+    idx = np.arange(lo, lo + len(d2))
+    A = np.zeros((len(idx), len(idx)))
+    for i, li in enumerate(idx):
+        for j, kj in enumerate(idx):
+            k = 2 * li - kj
+            if -15 <= k <= 15:
+                A[i, j] = a[k + 15]
+    assert np.abs(4 * A @ d2 - d2).max() < 1e-13
+
+
+def test_magic_filter_moments_match_scaling_function():
+    """sum_l l^k IMD[l] = int x^k phi(x) dx for k <= 7, with the moments of phi
+    obtained from the refinement relation of the low-pass filter."""
+    h = lp_taps()
+    j = np.arange(16) - 7.0
+    M = [1.0]
+    for k in range(1, 8):
+        s = sum(comb(k, m) * (h * j ** (k - m)).sum() * M[m] for m in range(k))
+        M.append(sqrt(2) / 2 ** (k + 1) * s / (1 - 2.0 ** -k))
+    lo, imd = O.taps(O.IMD)
+    l = np.arange(lo, lo + 16.0)
+    for k in range(8):
+        assert abs((l ** k * imd).sum() - M[k]) < 1e-11 * max(1, abs(M[k]))
+
+
+# --------------------------------------------------------------------------
+# golden vectors from the reference's FP64 code
+def test_golden_magicfilter(golden_dir):
+    g = np.load(os.path.join(golden_dir, 'mf.npz'))
+    n1, ndat = int(g['n1']), int(g['ndat'])
+    x, xs = np.asfortranarray(g['x']), np.asfortranarray(g['x_shrink'])
+    for kind, fil, bc, xin, nin, nout in (('per', O.MD, 0, x, n1, n1), ('per_t', O.IMD, 0, x, n1, n1),
+                                          ('free', O.MD, 1, x, n1, n1 + 15),
+                                          ('free_t', O.IMD, -1, xs, n1 + 15, n1)):
+        y = np.zeros((nout, ndat), order='F')
+        O.s0s0_1d(fil, 0, [n1, ndat], bc, [nin, ndat], [nout, ndat], 1, xin, y)
+        assert rel(y.T, g[kind]) < 1e-15, kind
+
+
+def test_golden_kinetic(golden_dir):
+    g = np.load(os.path.join(golden_dir, 'kinetic.npz'))
+    n, h, c = [int(v) for v in g['n']], g['h'], float(g['c'])
+    x = np.asfortranarray(g['x'])
+    a = [-0.5 / hh ** 2 for hh in h]
+    y = np.zeros(n, order='F')
+    O.s0s0_1ds(O.D2, n, [0, 0, 0], n, n, 1, a, c, 0.0, x, y)
+    assert rel(y, g['y']) < 1e-14
+    y2 = np.zeros(n, order='F')
+    dots = O.s0s0_1ds(O.D2, n, [0, 0, 0], n, n, 1, a, 0.0, 0.0, x, y2)
+    assert rel(y2, g['y_ekin']) < 1e-14
+    assert rel(dots, g['kstrten']) < 1e-13
+
+
+def test_golden_generated_idwt(golden_dir):
+    g = np.load(os.path.join(golden_dir, 'idwt.npz'))
+    for k in range(int(g['ncases'])):
+        c = [int(v) for v in g['c%d' % k]]
+        idim, bc, narr = c[:3]
+        n, nx, ny = c[3:6], c[6:9], c[9:12]
+        a, a_y = [float(v) for v in g['s%d' % k]]
+        y = np.asfortranarray(g['y0_%d' % k].copy())
+        O.s1s0_1d(O.SYM8, idim, n, bc, nx, ny, narr, np.asfortranarray(g['x%d' % k]), y, a, a_y)
+        assert rel(y, g['y%d' % k]) < 1e-15, (idim, bc, a, a_y)
+
+
+def test_golden_bigdft_synthesis_phase(golden_dir):
+    """BigDFT's own synthesis is the generated IDWT shifted by one sample
+    (BigDFT/LibconvBase.rb:90-96 shift_realspace)."""
+    g = np.load(os.path.join(golden_dir, 'bigdft_wavelet.npz'))
+    n, ndat = int(g['n']), int(g['ndat'])
+    x = np.asfortranarray(g['x'])
+    y = np.zeros((2 * n, ndat), order='F')
+    for j in range(ndat):  # one d=1 call per line: every wavelet dim counts pairs
+        col = np.zeros(2 * n)
+        O.s1s0_1d(O.SYM8, 0, [n], 0, [n], [n], 1, np.ascontiguousarray(x[:, j]), col)
+        y[:, j] = col
+    ref = g['synthesis_per'].T
+    assert rel(np.roll(y, 1, axis=0), ref) < 1e-14
+
+
+@pytest.mark.skipif(not R.available('libref_idwt.so'), reason='oracle/_ref not built here')
+def test_live_reference_idwt_random_sizes():
+    rng = np.random.default_rng(7)
+    for trial in range(12):
+        d = int(rng.integers(1, 4))
+        n = [int(v) for v in rng.integers(8, 14, d)]
+        idim = int(rng.integers(0, d))
+        bc = int(rng.choice([0, 1, -1]))
+        nx, ny = list(n), list(n)
+        ein, eout = O.wavelet_extents(O.SYM8, bc, n[idim])
+        nx[idim], ny[idim] = ein + int(rng.integers(0, 3)), eout + int(rng.integers(0, 3))
+        narr = int(rng.integers(1, 3))
+        nin = int(np.prod([2 * v for v in nx])) * narr
+        nout = int(np.prod([2 * v for v in ny])) * narr
+        x = np.asfortranarray(rng.uniform(-1, 1, nin))
+        y0 = rng.uniform(-1, 1, nout)
+        a, a_y = float(rng.uniform(0.5, 2)), float(rng.choice([0.0, 0.7]))
+        ya, yb = np.asfortranarray(y0.copy()), np.asfortranarray(y0.copy())
+        R.idwt_1d(d, idim, n, bc, nx, ny, narr, x, ya, a, a_y)
+        O.s1s0_1d(O.SYM8, idim, n, bc, nx, ny, narr, x, yb, a, a_y)
+        assert rel(yb, ya) < 1e-15
+
+
+# --------------------------------------------------------------------------
+# mathematics
+def _sweep(fil, bc, n, x, ndat=1, **kw):
+    nin, nout = O.conv_extents(fil, bc, n)
+    y = np.zeros(nout * ndat)
+    dot = O.s0s0_1d(fil, 0, [n, ndat], bc, [nin, ndat], [nout, ndat], 1, np.ascontiguousarray(x), y,
+                    want_dot=True, **kw)
+    return y, dot
+
+
+@pytest.mark.parametrize('fil', [O.MD, O.IMD, O.D2, O.D1])
+def test_plane_wave_eigenvalues(fil):
+    n = 40
+    lo, f = O.taps(fil)
+    l = np.arange(lo, lo + len(f))
+    for m in (1, 3, 7):
+        th = 2 * np.pi * m / n
+        j = np.arange(n)
+        lam = (f * np.exp(1j * th * l)).sum()
+        yc, _ = _sweep(fil, 0, n, np.cos(th * j))
+        ys, _ = _sweep(fil, 0, n, np.sin(th * j))
+        z = np.exp(1j * th * j) * lam
+        tol = 1e-14 * max(1.0, abs(lam))
+        assert np.abs(yc - z.real).max() < tol and np.abs(ys - z.imag).max() < tol
+
+
+def test_small_n_brute_force():
+    """n below the filter width: true periodic wrap and both-sided clamping."""
+    rng = np.random.default_rng(3)
+    for fil in (O.MD, O.D2):
+        lo, f = O.taps(fil)
+        up = lo + len(f) - 1
+        for n in (1, 2, 5, 9):
+            x = rng.uniform(-1, 1, n)
+            y, _ = _sweep(fil, 0, n, x)
+            ref = [sum(f[l - lo] * x[(i + l) % n] for l in range(lo, up + 1)) for i in range(n)]
+            assert rel(y, ref) < 1e-14
+            y, _ = _sweep(fil, 1, n, x)
+            ref = [sum(f[l - lo] * x[i + l] for l in range(lo, up + 1) if 0 <= i + l < n)
+                   for i in range(-up, n - lo)]
+            assert rel(y, ref) < 1e-14
+
+
+def test_adjoint_grow_shrink_and_periodic():
+    rng = np.random.default_rng(4)
+    n = 23
+    x = rng.uniform(-1, 1, n)
+    yg, _ = _sweep(O.MD, 1, n, x)
+    v = rng.uniform(-1, 1, n + 15)
+    ts, _ = _sweep(O.IMD, -1, n, v)
+    assert abs(np.dot(yg, v) - np.dot(x, ts)) < 1e-13
+    u = rng.uniform(-1, 1, n)
+    yp, _ = _sweep(O.MD, 0, n, x)
+    tp, _ = _sweep(O.IMD, 0, n, u)
+    assert abs(np.dot(yp, u) - np.dot(x, tp)) < 1e-13
+
+
+def test_epilogue_terms_and_dot():
+    rng = np.random.default_rng(5)
+    n, ndat = 31, 4
+    x = rng.uniform(-1, 1, n * ndat)
+    y0 = rng.uniform(-1, 1, n * ndat)
+    base = np.zeros(n * ndat)
+    O.s0s0_1d(O.D2, 0, [n, ndat], 0, [n, ndat], [n, ndat], 1, x, base)
+    y = y0.copy()
+    dot = O.s0s0_1d(O.D2, 0, [n, ndat], 0, [n, ndat], [n, ndat], 1, x, y, a=-2.5, a_x=0.75, a_y=0.5,
+                    want_dot=True)
+    assert rel(y, -2.5 * base + 0.75 * x + 0.5 * y0) < 1e-14
+    assert abs(dot - np.dot(x, -2.5 * base)) < 1e-12 * abs(dot)
+    # a_y == 0: y is not read (NaN stays out)
+    y = np.full(n * ndat, np.nan)
+    O.s0s0_1d(O.D2, 0, [n, ndat], 0, [n, ndat], [n, ndat], 1, x, y, a=1.0)
+    assert np.isfinite(y).all()
+
+
+def test_kinetic_quadratic_exact_in_shrink():
+    """D2 maps i^2 to 2 (sum f = 0, sum l f = 0, sum l^2 f = 2)."""
+    n = 12
+    j = np.arange(-14, n + 14, dtype=float)
+    y, dot = _sweep(O.D2, -1, n, 0.5 * j ** 2 - 3 * j + 1)
+    assert np.abs(y - 1.0).max() < 1e-11
+
+
+def test_kinetic_energy_parseval():
+    rng = np.random.default_rng(6)
+    n = [10, 12, 9]
+    x = np.asfortranarray(rng.uniform(-1, 1, n))
+    a = [-1.3, -0.7, -2.1]
+    y = np.zeros(n, order='F')
+    dots = O.s0s0_1ds(O.D2, n, [0, 0, 0], n, n, 1, a, 0.0, 0.0, x, y)
+    lo, f = O.taps(O.D2)
+    l = np.arange(lo, lo + len(f))
+    X = np.fft.fftn(x)
+    for dim in range(3):
+        th = 2 * np.pi * np.fft.fftfreq(n[dim])
+        lam = np.array([(f * np.cos(t * l)).sum() for t in th])
+        shape = [1, 1, 1]
+        shape[dim] = n[dim]
+        e = (a[dim] * lam.reshape(shape) * np.abs(X) ** 2).sum() / np.prod(n)
+        assert abs(dots[dim] - e) < 1e-12 * abs(e)
+
+
+@pytest.mark.parametrize('wav', [O.SYM8])
+def test_wavelet_round_trips(wav):
+    rng = np.random.default_rng(8)
+    n = [6, 9, 7]
+    s = np.asfortranarray(rng.uniform(-1, 1, [2 * v for v in n]))
+    c = np.zeros(s.shape, order='F')
+    O.s0s1_md(wav, n, [0, 0, 0], n, n, 1, s, c)
+    assert abs(np.linalg.norm(c) - np.linalg.norm(s)) < 1e-13
+    back = np.zeros(s.shape, order='F')
+    O.s1s0_md(wav, n, [0, 0, 0], n, n, 1, c, back)
+    assert rel(back, s) < 1e-14
+    # analysis grow then synthesis shrink is the identity
+    ng = [v + 7 for v in n]
+    cg = np.zeros([2 * v for v in ng], order='F')
+    O.s0s1_md(wav, n, [1, 1, 1], n, ng, 1, s, cg)
+    back = np.zeros(s.shape, order='F')
+    O.s1s0_md(wav, n, [-1, -1, -1], ng, n, 1, cg, back)
+    assert rel(back, s) < 1e-13
+
+
+def test_wavelet_vanishing_moments():
+    """Polynomials of degree < 8 give zero details away from the wrap."""
+    n = 40
+    j = np.arange(2 * n, dtype=float) / (2 * n)
+    for p in range(8):
+        s = np.ascontiguousarray(j ** p)
+        c = np.zeros(2 * n)
+        O.s0s1_1d(O.SYM8, 0, [n], 0, [n], [n], 1, s, c)
+        H = c[n:]
+        assert np.abs(H[8:n - 8]).max() < 1e-12, p
