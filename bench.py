@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the libconv hot path on one MI355X.
+
+Step = the 3D MagicFilter followed by the 3D Kinetic operator with its FP64
+kinetic energy (y1 = MF3D(x), y2 = K3D(y1), ekin_d = <y1, a_d D2_d y1>),
+a_d = -0.5/h^2 with h = 0.3, through the library's stream API on device-
+resident data.  Default workload: 256^3 output grid, free BC in shrink mode
+(x 299^3 -> y1 284^3 -> y2 256^3), FP64 and FP32 storage.  Synthetic input:
+U[-1, 1] (seeded) plus a Gaussian bump of amplitude 1 and sigma = 10 points.
+
+value = nominal output points (256^3) per second of the FP64 chain, in Gpts/s.
+Algorithmic bytes per step = s (|x| + 2 |y1| + |y2|) (each operator reads its
+input once and writes its output once).  Several buffer sets are rotated so
+that the working set of consecutive steps exceeds the 256 MB Infinity Cache.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'boast-lig_amd'))
+
+HBM_PEAK_GBS = 8000.0
+MALL_BYTES = 256 << 20
+H = 0.3
+
+
+def workload(name):
+    """(mf n, bc, x extent, y1 extent, kinetic n, y2 extent, nominal points)."""
+    if name == 'c4':      # 256^3 output, free shrink
+        n2 = 256
+        return dict(bc=-1, nx=n2 + 15 + 28, n1=n2 + 28, ny1=n2 + 28, n2=n2, ny2=n2, pts=n2 ** 3,
+                    desc='MF3D->K3D 256^3 output, free BC shrink (x 299^3, y1 284^3, y2 256^3)')
+    if name == 'c2p':     # 128^3 periodic
+        return dict(bc=0, nx=128, n1=128, ny1=128, n2=128, ny2=128, pts=128 ** 3,
+                    desc='MF3D->K3D 128^3, periodic BC')
+    if name == 'c2g':     # 128^3 free grow
+        return dict(bc=1, nx=128, n1=128, ny1=143, n2=143, ny2=171, pts=128 ** 3,
+                    desc='MF3D->K3D 128^3 input, free BC grow (y1 143^3, y2 171^3)')
+    raise SystemExit('unknown workload ' + name)
+
+
+def make_input(torch, w, dtype, seed=20250):
+    n = w['nx']
+    g = torch.Generator(device='cuda').manual_seed(seed)
+    x = torch.rand((n, n, n), dtype=torch.float64, device='cuda', generator=g) * 2 - 1
+    ax = torch.arange(n, dtype=torch.float64, device='cuda') - (n - 1) / 2
+    r2 = ax.view(-1, 1, 1) ** 2 + ax.view(1, -1, 1) ** 2 + ax.view(1, 1, -1) ** 2
+    x += torch.exp(-r2 / (2 * 10.0 ** 2))
+    # storage: Fortran order (dim 0 fastest) == the flat buffer of this tensor
+    return x.reshape(-1).to(dtype).contiguous()
+
+
+def chain_args(w):
+    bc = [w['bc']] * 3
+    mf = dict(n=[w['n1'] if w['bc'] != 1 else w['nx']] * 3, bc=bc, nx=[w['nx']] * 3, ny=[w['ny1']] * 3)
+    if w['bc'] == -1:
+        mf['n'] = [w['ny1']] * 3
+    kin = dict(n=[w['n2'] if w['bc'] != 1 else w['ny1']] * 3, bc=bc, nx=[w['ny1']] * 3, ny=[w['ny2']] * 3)
+    if w['bc'] == -1:
+        kin['n'] = [w['ny2']] * 3
+    return mf, kin
+
+
+def run_gpu(torch, lc, w, dtype, steps, warmup):
+    mf, kin = chain_args(w)
+    a = [-0.5 / H ** 2] * 3
+    s = 4 if dtype == torch.float32 else 8
+    set_bytes = s * (w['nx'] ** 3 + w['ny1'] ** 3 + w['ny2'] ** 3)
+    nsets = max(1, -(-3 * MALL_BYTES // set_bytes))
+    x0 = make_input(torch, w, dtype)
+    sets = []
+    for k in range(nsets):
+        x = x0 if k == 0 else x0.clone()
+        y1 = torch.empty(w['ny1'] ** 3, dtype=dtype, device='cuda')
+        y2 = torch.empty(w['ny2'] ** 3, dtype=dtype, device='cuda')
+        dot = torch.zeros(3, dtype=torch.float64, device='cuda')
+        sets.append((x, y1, y2, dot))
+    st = torch.cuda.current_stream()
+
+    def step(i, ev=None):
+        x, y1, y2, dot = sets[i % nsets]
+        if ev:
+            ev[0].record(st)
+        lc.conv_md(lc.FIL_MD, mf['n'], mf['bc'], mf['nx'], mf['ny'], 1, x, y1, stream=st)
+        if ev:
+            ev[1].record(st)
+        lc.conv_1ds(lc.FIL_D2, kin['n'], kin['bc'], kin['nx'], kin['ny'], 1, a, 0.0, 0.0, y1, y2,
+                    dot=dot, stream=st)
+        if ev:
+            ev[2].record(st)
+
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    # whole-step timing (no per-op events inside)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for i in range(steps):
+        step(i)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    # per-operator timing (events between the operators, same stream)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for i in range(steps):
+        step(i, evs[i])
+    torch.cuda.synchronize()
+    t_mf = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
+    t_k = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
+    x, y1, y2, dot = sets[0]
+    step(0)
+    torch.cuda.synchronize()
+    return dict(ms=ms, t_mf=t_mf, t_k=t_k, y1=y1.double().cpu().numpy(), y2=y2.double().cpu().numpy(),
+                ekin=dot.cpu().numpy(), x=x.double().cpu().numpy(), nsets=nsets)
+
+
+def errs(a, b):
+    import numpy as np
+    d = a - b
+    return dict(max_rel=float(np.abs(d).max() / np.abs(b).max()),
+                rms_rel=float(np.sqrt((d * d).sum() / (b * b).sum())))
+
+
+def oracle_chain(O, w, x):
+    import numpy as np
+    mf, kin = chain_args(w)
+    y1 = np.zeros(w['ny1'] ** 3)
+    O.s0s0_md(O.MD, mf['n'], mf['bc'], mf['nx'], mf['ny'], 1, x, y1)
+    y2 = np.zeros(w['ny2'] ** 3)
+    a = [-0.5 / H ** 2] * 3
+    dot = O.s0s0_1ds(O.D2, kin['n'], kin['bc'], kin['nx'], kin['ny'], 1, a, 0.0, 0.0, y1, y2)
+    return y1, y2, dot
+
+
+def cpu_baseline(O, np):
+    """The oracle's OpenMP chain on the 128^3-output shrink sample (x 171^3):
+    one warm-up, then the min of 3 runs (BigDFT/MagicFilter_test.rb:69-79
+    keeps the min of repeated runs)."""
+    n2 = 128
+    w = dict(bc=-1, nx=n2 + 43, n1=n2 + 28, ny1=n2 + 28, n2=n2, ny2=n2, pts=n2 ** 3)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, w['nx'] ** 3)
+    oracle_chain(O, w, x)
+    best = 1e30
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle_chain(O, w, x)
+        best = min(best, time.perf_counter() - t0)
+    return dict(value=w['pts'] / best / 1e9, unit='Gpts/s', cores=O.num_threads(), kind='port',
+                sample='MF3D->K3D chain at 128^3 output, free shrink (x 171^3), FP64, min of 3 after '
+                       '1 warm-up; oracle/liblc_oracle.so with OpenMP', seconds=best)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='c4', choices=['c4', 'c2p', 'c2g'])
+    ap.add_argument('--no-oracle', action='store_true', help='skip the FP64 oracle error check')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--fp64-only', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU; --gpus %d is not supported '
+              '(orbitals shard across GPUs without communication, see DESIGN.md)' % args.gpus)
+        return 2
+    import numpy as np
+    import torch
+    import libconv as lc
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    lc.lib()
+    w = workload(args.workload)
+    r64 = run_gpu(torch, lc, w, torch.float64, args.steps, args.warmup)
+    r32 = None if args.fp64_only else run_gpu(torch, lc, w, torch.float32, args.steps, args.warmup)
+
+    s = 8
+    bx, by1, by2 = w['nx'] ** 3, w['ny1'] ** 3, w['ny2'] ** 3
+    chain_bytes = s * (bx + 2 * by1 + by2)
+    mf_bytes = s * (bx + by1)
+    k_bytes = s * (by1 + by2)
+    gpts = w['pts'] / (r64['ms'] * 1e-3) / 1e9
+    # dominant operator of the step
+    dom = 'kinetic3d' if r64['t_k'] >= r64['t_mf'] else 'magicfilter3d'
+    dom_t = max(r64['t_k'], r64['t_mf'])
+    dom_b = k_bytes if dom == 'kinetic3d' else mf_bytes
+    achieved = dom_b / (dom_t * 1e-3) / 1e9
+    out = {
+        'metric': 'Gpts/s & % of HBM roofline, 3D MagicFilter+Kinetic FP64 vs FP32 at 1e-6',
+        'value': round(gpts, 4),
+        'unit': 'Gpts/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(r64['ms'], 5),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic: U[-1,1] seeded + Gaussian bump (amplitude 1, sigma 10 points)',
+        'config': {'workload': w['desc'], 'grid_out': [w['ny2']] * 3, 'h': H, 'bc': w['bc'],
+                   'buffer_sets': r64['nsets']},
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': dom,
+                     'avg_ms': round(dom_t, 5), 'algorithmic_bytes': dom_b},
+        'chain': {'GB_s': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9, 1),
+                  'frac_hbm': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  'algorithmic_bytes': chain_bytes, 'mf_ms': round(r64['t_mf'], 5),
+                  'kin_ms': round(r64['t_k'], 5)},
+    }
+    if r32 is not None:
+        g32 = w['pts'] / (r32['ms'] * 1e-3) / 1e9
+        out['fp32'] = {'value': round(g32, 4), 'ms_per_step': round(r32['ms'], 5),
+                       'ratio_vs_fp64': round(g32 / gpts, 3),
+                       'chain_GB_s': round(chain_bytes / 2 / (r32['ms'] * 1e-3) / 1e9, 1),
+                       'mf_ms': round(r32['t_mf'], 5), 'kin_ms': round(r32['t_k'], 5)}
+    if not args.no_oracle:
+        import oracle as O
+        y1r, y2r, dr = oracle_chain(O, w, r64['x'])
+        out['errors_fp64'] = {'y1': errs(r64['y1'], y1r), 'y2': errs(r64['y2'], y2r),
+                              'ekin_rel': float(abs(r64['ekin'].sum() - dr.sum()) / abs(dr.sum()))}
+        if r32 is not None:
+            # the FP32 run starts from the FP32-rounded input: compare with the
+            # FP64 oracle on the same (rounded) input
+            y1s, y2s, ds = oracle_chain(O, w, r32['x'])
+            out['fp32']['errors'] = {'y1': errs(r32['y1'], y1s), 'y2': errs(r32['y2'], y2s),
+                                     'ekin_rel': float(abs(r32['ekin'].sum() - ds.sum()) / abs(ds.sum()))}
+    if not args.no_cpu:
+        import oracle as O
+        out['cpu_baseline'] = cpu_baseline(O, np)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

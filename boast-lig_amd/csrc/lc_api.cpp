@@ -1,0 +1,776 @@
+// lc_api.cpp -- C ABI of the libconv HIP library (include/libconv.h).
+//
+// Dispatch mirrors the generated libconv procedures:
+//   1D  GenericConvolutionOperator1d#procedure (BigDFT/GenericConvolution2.rb
+//       :1560-1766): ndat_left/right from nx[] of the other dims (x2 per dim
+//       for wavelets), narr arrays at strides nti / nto, a == 1 / a_x == 0 /
+//       a_y == 0 select the variant without that term;
+//   md  GenericConvolutionOperator#procedure (:2131-2245): sweep i reads the
+//       array whose dims < i carry the output extents ny[] and dims >= i the
+//       input extents nx[].
+// Pointers are classified per call: device memory runs in place on the
+// default stream, host memory is staged through a per-thread device pool.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/libconv.h"
+#include "lc_kernels.hpp"
+
+using namespace lc;
+
+namespace {
+
+thread_local int g_err = LC_OK;
+thread_local char g_msg[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  g_err = code;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+  va_end(ap);
+  const char* ab = getenv("LIBCONV_ABORT_ON_ERROR");
+  if (ab && ab[0] == '1') {
+    fprintf(stderr, "libconv: %s\n", g_msg);
+    abort();
+  }
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail(LC_ERR_DEVICE, "%s: %s", where, hipGetErrorString(e));
+}
+
+#define HIPCHK(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+// ---------------------------------------------------------------- pools
+struct Pool {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+constexpr int kMaxDev = 16;
+constexpr int kSlots = 4;  // 0 scratch, 1 x staging, 2 y staging, 3 dot
+thread_local Pool g_pool[kMaxDev][kSlots];
+
+int pool_get(int slot, size_t bytes, void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(LC_ERR_UNSUPPORTED, "device %d out of range", dev);
+  Pool& pl = g_pool[dev][slot];
+  if (bytes == 0) bytes = 256;
+  if (pl.bytes < bytes) {
+    if (pl.p) {
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipFree(pl.p));
+      pl.p = nullptr;
+      pl.bytes = 0;
+    }
+    size_t want = std::max(bytes, pl.bytes + pl.bytes / 2);
+    hipError_t e = hipMalloc(&pl.p, want);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      e = hipMalloc(&pl.p, bytes);
+      if (e != hipSuccess) return fail(LC_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+      want = bytes;
+    }
+    pl.bytes = want;
+  }
+  *out = pl.p;
+  return LC_OK;
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t at;
+  hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+// Host-or-device array argument of a drop-in call.
+template <typename R>
+struct Arg {
+  const R* host_in = nullptr;
+  R* host_out = nullptr;
+  R* dev = nullptr;
+  size_t count = 0;
+};
+
+template <typename R>
+int stage_in(const R* p, size_t count, int slot, bool copy, hipStream_t st, Arg<R>* a) {
+  a->count = count;
+  if (is_device_ptr(p)) {
+    a->dev = const_cast<R*>(p);
+    return LC_OK;
+  }
+  void* d;
+  int rc = pool_get(slot, count * sizeof(R), &d);
+  if (rc) return rc;
+  a->dev = static_cast<R*>(d);
+  a->host_in = p;
+  if (copy && count) HIPCHK(hipMemcpyAsync(d, p, count * sizeof(R), hipMemcpyHostToDevice, st));
+  return LC_OK;
+}
+
+template <typename R>
+int stage_out(Arg<R>& a, R* host, hipStream_t st) {
+  if (a.host_in || host) {
+    if (a.count && !is_device_ptr(host))
+      HIPCHK(hipMemcpyAsync(host, a.dev, a.count * sizeof(R), hipMemcpyDeviceToHost, st));
+  }
+  return LC_OK;
+}
+
+// ---------------------------------------------------------------- geometry
+bool valid_bc(int bc) { return bc == 0 || bc == 1 || bc == -1; }
+
+int conv_window(int fil) {
+  switch (fil) {
+    case OP_MD: case OP_IMD: return 16;
+    case OP_D2: case OP_D1: return 29;
+  }
+  return -1;
+}
+
+int wav_window(int wav) {
+  switch (wav) {
+    case WAV_SYM8: return 8;
+    case WAV_SYMICOMB8: return 16;
+  }
+  return -1;
+}
+
+// required extents of the processed dim for window W (steps)
+int check_line(int bc, int n, int nx, int ny, int W, const char* who) {
+  if (!valid_bc(bc)) return fail(LC_ERR_ARG, "%s: bc %d not in {0, 1, -1}", who, bc);
+  if (n < 1) return fail(LC_ERR_ARG, "%s: n = %d < 1", who, n);
+  const int need_in = bc == -1 ? n + W - 1 : n;
+  const int need_out = bc == 1 ? n + W - 1 : n;
+  if (nx < need_in) return fail(LC_ERR_ARG, "%s: nx = %d < %d required by bc %d", who, nx, need_in, bc);
+  if (ny < need_out) return fail(LC_ERR_ARG, "%s: ny = %d < %d required by bc %d", who, ny, need_out, bc);
+  return LC_OK;
+}
+
+int check_dims(int d, const int* n, const int* nx, const int* ny, int narr, const char* who) {
+  if (d < 1 || d > 16) return fail(LC_ERR_ARG, "%s: d = %d not in [1, 16]", who, d);
+  if (!n || !nx || !ny) return fail(LC_ERR_ARG, "%s: null size array", who);
+  if (narr < 0) return fail(LC_ERR_ARG, "%s: narr = %d < 0", who, narr);
+  for (int i = 0; i < d; ++i)
+    if (n[i] < 1 || nx[i] < 1 || ny[i] < 1)
+      return fail(LC_ERR_ARG, "%s: dim %d has n/nx/ny = %d/%d/%d", who, i, n[i], nx[i], ny[i]);
+  return LC_OK;
+}
+
+// ---------------------------------------------------------------- 1D core
+// kind: KCONV (id = filter), KDWT / KIDWT (id = wavelet)
+template <typename R>
+int run_1d(int kind, int id, int d, int idim, const int* n, int bc, const int* nx, const int* ny,
+           int narr, const R* x, R* y, double a, double a_x, double a_y, double* dot_host,
+           double* dot_dev_user, hipStream_t st, bool device_api) {
+  const char* who = kind == KCONV ? "s0s0_1d" : (kind == KDWT ? "s0s1_1d" : "s1s0_1d");
+  int rc = check_dims(d, n, nx, ny, narr, who);
+  if (rc) return rc;
+  if (idim < 0 || idim >= d) return fail(LC_ERR_ARG, "%s: idim = %d not in [0, %d)", who, idim, d);
+  const int W = kind == KCONV ? conv_window(id) : wav_window(id);
+  if (W < 0) return fail(LC_ERR_ARG, "%s: unknown operator id %d", who, id);
+  rc = check_line(bc, n[idim], nx[idim], ny[idim], W, who);
+  if (rc) return rc;
+  const int mul = kind == KCONV ? 1 : 2;
+  long L = 1, Rr = 1;
+  for (int i = 0; i < idim; ++i) L *= (long)mul * nx[i];
+  for (int i = idim + 1; i < d; ++i) Rr *= (long)mul * nx[i];
+  OpGeom g = kind == KCONV ? conv_geom(id) : wavelet_geom(id, kind == KIDWT, bc);
+  SweepArgs<R> p{};
+  p.L = L;
+  p.nr = Rr;
+  p.narr = narr;
+  p.ncols = L * Rr * narr;
+  p.n = n[idim];
+  p.nx = nx[idim];
+  p.ny = ny[idim];
+  p.line_in = L * mul * nx[idim];
+  p.line_out = L * mul * ny[idim];
+  p.nti = p.line_in * Rr;
+  p.nto = p.line_out * Rr;
+  p.ox = bc == -1 ? g.lo : 0;
+  p.oy = bc == 1 ? -g.up : 0;
+  p.i_first = bc == 1 ? -g.up : 0;
+  p.nout = bc == 1 ? n[idim] + W - 1 : n[idim];
+  p.a = (R)a;
+  p.a_x = (R)a_x;
+  p.a_y = (R)a_y;
+  p.flags = (a != 1.0 ? F_A : 0) | (kind == KCONV && a_x != 0.0 ? F_AX : 0) | (a_y != 0.0 ? F_AY : 0);
+  const bool want_dot = kind == KCONV && (dot_host || dot_dev_user);
+  if (want_dot) p.flags |= F_DOT;
+
+  Arg<R> ax, ay;
+  const size_t nin = (size_t)p.nti * narr, nout = (size_t)p.nto * narr;
+  if (device_api) {
+    ax.dev = const_cast<R*>(x);
+    ay.dev = y;
+  } else {
+    rc = stage_in(x, nin, 1, true, st, &ax);
+    if (rc) return rc;
+    rc = stage_in<R>(y, nout, 2, true, st, &ay);  // keeps padding of y intact
+    if (rc) return rc;
+  }
+  p.x = ax.dev;
+  p.y = ay.dev;
+  double* dot_dev = dot_dev_user;
+  if (want_dot) {
+    const long np = sweep_partials_needed(p, W);
+    void* sc;
+    rc = pool_get(0, sizeof(double) * (size_t)(np + 8), &sc);
+    if (rc) return rc;
+    p.dot_part = static_cast<double*>(sc);
+    if (!dot_dev) dot_dev = static_cast<double*>(sc) + np;
+  }
+  hipError_t e = sweep<R>(kind, id, bc, p, st, want_dot ? dot_dev : nullptr);
+  if (e != hipSuccess) return hip_fail(e, who);
+  if (!device_api) {
+    if (ay.host_in) HIPCHK(hipMemcpyAsync(y, ay.dev, nout * sizeof(R), hipMemcpyDeviceToHost, st));
+    if (dot_host) HIPCHK(hipMemcpyAsync(dot_host, dot_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return LC_OK;
+}
+
+// ---------------------------------------------------------------- md core
+__global__ void axpy_kernel_d(long n, double a, const double* x, double* y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = __builtin_fma(x[i], a, y[i]);
+}
+__global__ void axpy_kernel_f(long n, float a, const float* x, float* y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = __builtin_fma(x[i], a, y[i]);
+}
+hipError_t axpy(long n, double a, const double* x, double* y, hipStream_t st) {
+  if (n > 0) axpy_kernel_d<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, x, y);
+  return hipGetLastError();
+}
+hipError_t axpy(long n, float a, const float* x, float* y, hipStream_t st) {
+  if (n > 0) axpy_kernel_f<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, x, y);
+  return hipGetLastError();
+}
+
+// kind: KCONV separable product / KDWT / KIDWT over all d dims
+template <typename R>
+int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, const int* ny,
+           int narr, const R* x, R* y, double a, double a_x, double a_y, hipStream_t st,
+           bool device_api) {
+  const char* who = kind == KCONV ? "s0s0_md" : (kind == KDWT ? "s0s1_md" : "s1s0_md");
+  int rc = check_dims(d, n, nx, ny, narr, who);
+  if (rc) return rc;
+  if (!bc) return fail(LC_ERR_ARG, "%s: null bc", who);
+  const int W = kind == KCONV ? conv_window(id) : wav_window(id);
+  if (W < 0) return fail(LC_ERR_ARG, "%s: unknown operator id %d", who, id);
+  for (int i = 0; i < d; ++i) {
+    rc = check_line(bc[i], n[i], nx[i], ny[i], W, who);
+    if (rc) return rc;
+  }
+  const int mul = kind == KCONV ? 1 : 2;
+  long tin = 1, tout = 1;
+  for (int i = 0; i < d; ++i) {
+    tin *= (long)mul * nx[i];
+    tout *= (long)mul * ny[i];
+  }
+  if (a_x != 0.0 && tin != tout)
+    return fail(LC_ERR_ARG, "%s: a_x needs input and output of the same shape", who);
+  Arg<R> ax, ay;
+  if (device_api) {
+    ax.dev = const_cast<R*>(x);
+    ay.dev = y;
+  } else {
+    rc = stage_in(x, (size_t)tin * narr, 1, true, st, &ax);
+    if (rc) return rc;
+    rc = stage_in<R>(y, (size_t)tout * narr, 2, true, st, &ay);
+    if (rc) return rc;
+  }
+  // intermediates: after sweep s the dims < s+1 carry ny
+  int cur[16];
+  for (int i = 0; i < d; ++i) cur[i] = nx[i];
+  size_t wmax = 0;
+  for (int s = 0; s < d - 1; ++s) {
+    cur[s] = ny[s];
+    size_t t = 1;
+    for (int i = 0; i < d; ++i) t *= (size_t)mul * cur[i];
+    wmax = std::max(wmax, t);
+  }
+  R* w[2] = {nullptr, nullptr};
+  if (d > 1) {
+    void* sc;
+    rc = pool_get(0, 2 * wmax * (size_t)std::max(narr, 1) * sizeof(R), &sc);
+    if (rc) return rc;
+    w[0] = static_cast<R*>(sc);
+    w[1] = w[0] + wmax * (size_t)std::max(narr, 1);
+  }
+  for (int i = 0; i < d; ++i) cur[i] = nx[i];
+  const R* src = ax.dev;
+  for (int s = 0; s < d; ++s) {
+    const bool last = s == d - 1;
+    R* dst = last ? ay.dev : w[s & 1];
+    OpGeom g = kind == KCONV ? conv_geom(id) : wavelet_geom(id, kind == KIDWT, bc[s]);
+    SweepArgs<R> p{};
+    long L = 1, Rr = 1;
+    for (int i = 0; i < s; ++i) L *= (long)mul * cur[i];
+    for (int i = s + 1; i < d; ++i) Rr *= (long)mul * cur[i];
+    p.x = src;
+    p.y = dst;
+    p.L = L;
+    p.nr = Rr;
+    p.narr = narr;
+    p.ncols = L * Rr * narr;
+    p.n = n[s];
+    p.nx = cur[s];
+    p.ny = ny[s];
+    p.line_in = L * mul * cur[s];
+    p.line_out = L * mul * ny[s];
+    p.nti = p.line_in * Rr;
+    p.nto = p.line_out * Rr;
+    p.ox = bc[s] == -1 ? g.lo : 0;
+    p.oy = bc[s] == 1 ? -g.up : 0;
+    p.i_first = bc[s] == 1 ? -g.up : 0;
+    p.nout = bc[s] == 1 ? n[s] + W - 1 : n[s];
+    p.a = last ? (R)a : R(1);
+    p.a_y = (R)a_y;
+    p.flags = (last && a != 1.0 ? F_A : 0) | (last && a_y != 0.0 ? F_AY : 0);
+    hipError_t e = sweep<R>(kind, id, bc[s], p, st, nullptr);
+    if (e != hipSuccess) return hip_fail(e, who);
+    cur[s] = ny[s];
+    src = dst;
+  }
+  if (a_x != 0.0) {
+    hipError_t e = axpy((long)tout * narr, (R)a_x, ax.dev, ay.dev, st);
+    if (e != hipSuccess) return hip_fail(e, who);
+  }
+  if (!device_api) {
+    if (ay.host_in) HIPCHK(hipMemcpyAsync(y, ay.dev, (size_t)tout * narr * sizeof(R), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return LC_OK;
+}
+
+// kinetic-style sum over d <= 3 dims
+template <typename R>
+int run_1ds(int fil, int d, const int* n, const int* bc, const int* nx, const int* ny, int narr,
+            const double* a, double a_x, double a_y, const R* x, R* y, double* dot_host,
+            double* dot_dev_user, hipStream_t st, bool device_api) {
+  const char* who = "s0s0_1ds";
+  int rc = check_dims(d, n, nx, ny, narr, who);
+  if (rc) return rc;
+  if (d > 3) return fail(LC_ERR_UNSUPPORTED, "%s: d = %d > 3 not supported on the GPU", who, d);
+  if (!bc || !a) return fail(LC_ERR_ARG, "%s: null bc or a", who);
+  const int W = conv_window(fil);
+  if (W < 0) return fail(LC_ERR_ARG, "%s: unknown filter %d", who, fil);
+  Box3 b;
+  double a3[3] = {0, 0, 0};
+  for (int i = 0; i < 3; ++i) {
+    if (i < d) {
+      rc = check_line(bc[i], n[i], nx[i], ny[i], W, who);
+      if (rc) return rc;
+      b.n[i] = n[i]; b.bc[i] = bc[i]; b.nx[i] = nx[i]; b.ny[i] = ny[i];
+      a3[i] = a[i];
+    } else {
+      b.n[i] = 1; b.bc[i] = 0; b.nx[i] = 1; b.ny[i] = 1;
+    }
+  }
+  long tin = (long)b.nx[0] * b.nx[1] * b.nx[2], tout = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  Arg<R> ax, ay;
+  if (device_api) {
+    ax.dev = const_cast<R*>(x);
+    ay.dev = y;
+  } else {
+    rc = stage_in(x, (size_t)tin * narr, 1, true, st, &ax);
+    if (rc) return rc;
+    rc = stage_in<R>(y, (size_t)tout * narr, 2, true, st, &ay);
+    if (rc) return rc;
+  }
+  const bool want_dot = dot_host || dot_dev_user;
+  double* dot3 = nullptr;
+  double* scratch = nullptr;
+  size_t sbytes = 0;
+  if (want_dot) {
+    sbytes = sum3d_scratch_bytes<R>(b, narr);
+    void* sc;
+    rc = pool_get(0, sbytes + 8 * sizeof(double), &sc);
+    if (rc) return rc;
+    scratch = static_cast<double*>(sc);
+    dot3 = reinterpret_cast<double*>(static_cast<char*>(sc) + sbytes);
+  }
+  hipError_t e = sum3d<R>(fil, b, narr, ax.dev, ay.dev, a3, (R)a_x, (R)a_y, want_dot ? dot3 : nullptr,
+                          st, scratch, sbytes);
+  if (e != hipSuccess) return hip_fail(e, who);
+  if (want_dot && dot_dev_user)
+    HIPCHK(hipMemcpyAsync(dot_dev_user, dot3, d * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (!device_api) {
+    if (ay.host_in) HIPCHK(hipMemcpyAsync(y, ay.dev, (size_t)tout * narr * sizeof(R), hipMemcpyDeviceToHost, st));
+    if (dot_host) HIPCHK(hipMemcpyAsync(dot_host, dot3, d * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return LC_OK;
+}
+
+void kin_extents(int d, const int* n, const int* bc, int* nx, int* ny) {
+  for (int i = 0; i < d && i < 16; ++i) {
+    nx[i] = bc[i] == -1 ? n[i] + 28 : n[i];
+    ny[i] = bc[i] == 1 ? n[i] + 28 : n[i];
+  }
+}
+
+void wav_extents(int d, const int* n, const int* bc, int recompose, int* nx, int* ny) {
+  (void)recompose;
+  for (int i = 0; i < d && i < 16; ++i) {
+    nx[i] = bc[i] == -1 ? n[i] + 7 : n[i];
+    ny[i] = bc[i] == 1 ? n[i] + 7 : n[i];
+  }
+}
+
+hipStream_t hs(void* s) { return static_cast<hipStream_t>(s); }
+
+long cost_1d(int d, int idim, const int* n, int narr, int per_step, int mul) {
+  long ndat = 1;
+  for (int i = 0; i < d; ++i)
+    if (i != idim) ndat *= (long)mul * n[i];
+  return (long)narr * n[idim] * ndat * per_step;
+}
+
+}  // namespace
+
+namespace lc {
+OpGeom conv_geom(int id) {
+  switch (id) {
+    case OP_MD: return {FMD::LO, FMD::UP, FMD::N};
+    case OP_IMD: return {FIMD::LO, FIMD::UP, FIMD::N};
+    case OP_D2: return {FD2::LO, FD2::UP, FD2::N};
+    case OP_D1: return {FD1::LO, FD1::UP, FD1::N};
+  }
+  return {0, 0, -1};
+}
+OpGeom wavelet_geom(int id, int recompose, int bc) {
+  if (id == WAV_SYM8)
+    return recompose ? OpGeom{Sym8Rec::LO, Sym8Rec::UP, Sym8Rec::W} : OpGeom{Sym8Dec::LO, Sym8Dec::UP, Sym8Dec::W};
+  if (bc == 0)
+    return recompose ? OpGeom{CombPRec::LO, CombPRec::UP, CombPRec::W} : OpGeom{CombPDec::LO, CombPDec::UP, CombPDec::W};
+  return recompose ? OpGeom{CombGRec::LO, CombGRec::UP, CombGRec::W} : OpGeom{CombGDec::LO, CombGDec::UP, CombGDec::W};
+}
+}  // namespace lc
+
+// ====================================================================== C ABI
+extern "C" {
+
+int lc_last_error(void) {
+  int e = g_err;
+  g_err = LC_OK;
+  return e;
+}
+const char* lc_last_error_message(void) { return g_msg; }
+const char* lc_version(void) { return "libconv-mi355x 0.1 (gfx950)"; }
+
+int lc_extent(int op, int bc, int n, int side) {
+  int W;
+  if (op >= 0 && op < 4) W = conv_window(op);
+  else if (op >= 10 && op < 12) W = wav_window(op - 10);
+  else if (op >= 20 && op < 22) W = wav_window(op - 20);
+  else return -1;
+  if (!valid_bc(bc) || n < 1) return -1;
+  if (side == 0) return bc == -1 ? n + W - 1 : n;
+  return bc == 1 ? n + W - 1 : n;
+}
+
+// ---------------------------------------------------------------- 1D C names
+#define LC_CONV1D(NAME, R, FIL)                                                                  \
+  void NAME(int d, int idim, const int* n, int bc, const int* nx, const int* ny, int narr,       \
+            const R* x, R* y, R a, R a_x, R a_y) {                                               \
+    run_1d<R>(KCONV, FIL, d, idim, n, bc, nx, ny, narr, x, y, a, a_x, a_y, nullptr, nullptr,     \
+              nullptr, false);                                                                   \
+  }
+LC_CONV1D(d_s0s0_1d_sym8_md, double, OP_MD)
+LC_CONV1D(s_s0s0_1d_sym8_md, float, OP_MD)
+LC_CONV1D(d_s0s0_1d_sym8_imd, double, OP_IMD)
+LC_CONV1D(s_s0s0_1d_sym8_imd, float, OP_IMD)
+LC_CONV1D(d_s0s0_1d_sym8_d2, double, OP_D2)
+LC_CONV1D(s_s0s0_1d_sym8_d2, float, OP_D2)
+LC_CONV1D(d_s0s0_1d_sym8_d1, double, OP_D1)
+LC_CONV1D(s_s0s0_1d_sym8_d1, float, OP_D1)
+#undef LC_CONV1D
+
+void d_s0s0_1d_sym8_d2_dotin(int d, int idim, const int* n, int bc, const int* nx, const int* ny,
+                             int narr, const double* x, double* y, double a, double a_x, double a_y,
+                             double* dot_in) {
+  double dot = 0.0;
+  if (run_1d<double>(KCONV, OP_D2, d, idim, n, bc, nx, ny, narr, x, y, a, a_x, a_y, &dot, nullptr,
+                     nullptr, false) == LC_OK && dot_in)
+    *dot_in = dot;
+}
+void s_s0s0_1d_sym8_d2_dotin(int d, int idim, const int* n, int bc, const int* nx, const int* ny,
+                             int narr, const float* x, float* y, float a, float a_x, float a_y,
+                             float* dot_in) {
+  double dot = 0.0;
+  if (run_1d<float>(KCONV, OP_D2, d, idim, n, bc, nx, ny, narr, x, y, a, a_x, a_y, &dot, nullptr,
+                    nullptr, false) == LC_OK && dot_in)
+    *dot_in = (float)dot;
+}
+void s_s0s0_1d_sym8_d2_dotin64(int d, int idim, const int* n, int bc, const int* nx, const int* ny,
+                               int narr, const float* x, float* y, float a, float a_x, float a_y,
+                               double* dot_in) {
+  double dot = 0.0;
+  if (run_1d<float>(KCONV, OP_D2, d, idim, n, bc, nx, ny, narr, x, y, a, a_x, a_y, &dot, nullptr,
+                    nullptr, false) == LC_OK && dot_in)
+    *dot_in = dot;
+}
+
+#define LC_WAV1D(NAME, R, KIND, WAV)                                                             \
+  void NAME(int d, int idim, const int* n, int bc, const int* nx, const int* ny, int narr,       \
+            const R* x, R* y, R a, R a_y) {                                                      \
+    run_1d<R>(KIND, WAV, d, idim, n, bc, nx, ny, narr, x, y, a, 0.0, a_y, nullptr, nullptr,      \
+              nullptr, false);                                                                   \
+  }
+LC_WAV1D(d_s0s1_1d_sym8, double, KDWT, WAV_SYM8)
+LC_WAV1D(s_s0s1_1d_sym8, float, KDWT, WAV_SYM8)
+LC_WAV1D(d_s1s0_1d_sym8, double, KIDWT, WAV_SYM8)
+LC_WAV1D(s_s1s0_1d_sym8, float, KIDWT, WAV_SYM8)
+LC_WAV1D(d_s0s1_1d_symicomb8, double, KDWT, WAV_SYMICOMB8)
+LC_WAV1D(s_s0s1_1d_symicomb8, float, KDWT, WAV_SYMICOMB8)
+LC_WAV1D(d_s1s0_1d_symicomb8, double, KIDWT, WAV_SYMICOMB8)
+LC_WAV1D(s_s1s0_1d_symicomb8, float, KIDWT, WAV_SYMICOMB8)
+#undef LC_WAV1D
+
+void d_s0s0_1d_sym8_md_cost(int d, int idim, const int* n, int bc, const int* nx, const int* ny,
+                            int narr, const double* x, double* y, double a, double a_x, double a_y,
+                            int* cost) {
+  (void)bc; (void)nx; (void)ny; (void)x; (void)y; (void)a; (void)a_x; (void)a_y;
+  if (cost) *cost = (int)cost_1d(d, idim, n, narr, 32, 1);
+}
+void d_s0s0_1d_sym8_d2_dotin_cost(int d, int idim, const int* n, int bc, const int* nx,
+                                  const int* ny, int narr, const double* x, double* y, double a,
+                                  double a_x, double a_y, double* dot_in, int* cost) {
+  (void)bc; (void)nx; (void)ny; (void)x; (void)y; (void)a; (void)a_x; (void)a_y; (void)dot_in;
+  if (cost) *cost = (int)cost_1d(d, idim, n, narr, 58, 1);
+}
+void d_s0s1_1d_sym8_cost(int d, int idim, const int* n, int bc, const int* nx, const int* ny,
+                         int narr, const double* x, double* y, double a, double a_y, int* cost) {
+  (void)bc; (void)nx; (void)ny; (void)x; (void)y; (void)a; (void)a_y;
+  if (cost) *cost = (int)cost_1d(d, idim, n, narr, 64, 2);
+}
+void d_s1s0_1d_sym8_cost(int d, int idim, const int* n, int bc, const int* nx, const int* ny,
+                         int narr, const double* x, double* y, double a, double a_y, int* cost) {
+  (void)bc; (void)nx; (void)ny; (void)x; (void)y; (void)a; (void)a_y;
+  if (cost) *cost = (int)cost_1d(d, idim, n, narr, 64, 2);
+}
+
+// ---------------------------------------------------------------- md C names
+void d_sym8_md_ld_narr(int d, const int* n, const int* bc, const int* nx, const int* ny, int narr,
+                       const double* x, double* y, double* w1, double* w2) {
+  (void)w1; (void)w2;
+  run_md<double>(KCONV, OP_MD, d, n, bc, nx, ny, narr, x, y, 1.0, 0.0, 0.0, nullptr, false);
+}
+void s_sym8_md_ld_narr(int d, const int* n, const int* bc, const int* nx, const int* ny, int narr,
+                       const float* x, float* y, float* w1, float* w2) {
+  (void)w1; (void)w2;
+  run_md<float>(KCONV, OP_MD, d, n, bc, nx, ny, narr, x, y, 1.0, 0.0, 0.0, nullptr, false);
+}
+void d_sym8_imd_ld_narr(int d, const int* n, const int* bc, const int* nx, const int* ny, int narr,
+                        const double* x, double* y, double* w1, double* w2) {
+  (void)w1; (void)w2;
+  run_md<double>(KCONV, OP_IMD, d, n, bc, nx, ny, narr, x, y, 1.0, 0.0, 0.0, nullptr, false);
+}
+void s_sym8_imd_ld_narr(int d, const int* n, const int* bc, const int* nx, const int* ny, int narr,
+                        const float* x, float* y, float* w1, float* w2) {
+  (void)w1; (void)w2;
+  run_md<float>(KCONV, OP_IMD, d, n, bc, nx, ny, narr, x, y, 1.0, 0.0, 0.0, nullptr, false);
+}
+
+void d_sym8_d2(int d, const int* n, const int* bc, const double* x, double* y, double* w1, double* w2,
+               const double* a, double a_x, double a_y) {
+  (void)w1; (void)w2;
+  if (d < 1 || d > 16 || !n || !bc) { fail(LC_ERR_ARG, "sym8_d2: bad d or null n/bc"); return; }
+  int nx[16], ny[16];
+  kin_extents(d, n, bc, nx, ny);
+  run_1ds<double>(OP_D2, d, n, bc, nx, ny, 1, a, a_x, a_y, x, y, nullptr, nullptr, nullptr, false);
+}
+void s_sym8_d2(int d, const int* n, const int* bc, const float* x, float* y, float* w1, float* w2,
+               const float* a, float a_x, float a_y) {
+  (void)w1; (void)w2;
+  if (d < 1 || d > 16 || !n || !bc || !a) { fail(LC_ERR_ARG, "sym8_d2: bad d or null n/bc/a"); return; }
+  int nx[16], ny[16];
+  kin_extents(d, n, bc, nx, ny);
+  double ad[16];
+  for (int i = 0; i < d; ++i) ad[i] = a[i];
+  run_1ds<float>(OP_D2, d, n, bc, nx, ny, 1, ad, a_x, a_y, x, y, nullptr, nullptr, nullptr, false);
+}
+void d_sym8_d2_dotin(int d, const int* n, const int* bc, const double* x, double* y, double* w1,
+                     double* w2, const double* a, double a_x, double a_y, double* dot_in) {
+  (void)w1; (void)w2;
+  if (d < 1 || d > 16 || !n || !bc) { fail(LC_ERR_ARG, "sym8_d2: bad d or null n/bc"); return; }
+  int nx[16], ny[16];
+  kin_extents(d, n, bc, nx, ny);
+  double dot[16] = {0};
+  if (run_1ds<double>(OP_D2, d, n, bc, nx, ny, 1, a, a_x, a_y, x, y, dot, nullptr, nullptr, false) == LC_OK && dot_in)
+    for (int i = 0; i < d; ++i) dot_in[i] = dot[i];
+}
+void s_sym8_d2_dotin64(int d, const int* n, const int* bc, const float* x, float* y, float* w1,
+                       float* w2, const float* a, float a_x, float a_y, double* dot_in) {
+  (void)w1; (void)w2;
+  if (d < 1 || d > 16 || !n || !bc || !a) { fail(LC_ERR_ARG, "sym8_d2: bad d or null n/bc/a"); return; }
+  int nx[16], ny[16];
+  kin_extents(d, n, bc, nx, ny);
+  double ad[16], dot[16] = {0};
+  for (int i = 0; i < d; ++i) ad[i] = a[i];
+  if (run_1ds<float>(OP_D2, d, n, bc, nx, ny, 1, ad, a_x, a_y, x, y, dot, nullptr, nullptr, false) == LC_OK && dot_in)
+    for (int i = 0; i < d; ++i) dot_in[i] = dot[i];
+}
+
+#define LC_WAVMD(NAME, R, KIND)                                                                  \
+  void NAME(int d, const int* n, const int* bc, const R* x, R* y, R* w1, R* w2) {                 \
+    (void)w1; (void)w2;                                                                          \
+    if (d < 1 || d > 16 || !n || !bc) { fail(LC_ERR_ARG, #NAME ": bad d or null n/bc"); return; } \
+    int nx[16], ny[16];                                                                          \
+    wav_extents(d, n, bc, KIND == KIDWT, nx, ny);                                                \
+    run_md<R>(KIND, WAV_SYM8, d, n, bc, nx, ny, 1, x, y, 1.0, 0.0, 0.0, nullptr, false);          \
+  }
+LC_WAVMD(d_dwt_sym8, double, KDWT)
+LC_WAVMD(s_dwt_sym8, float, KDWT)
+LC_WAVMD(d_idwt_sym8, double, KIDWT)
+LC_WAVMD(s_idwt_sym8, float, KIDWT)
+#undef LC_WAVMD
+
+// ---------------------------------------------------------------- Fortran names
+void s0s1_1d_sym8_lp_d_(const int* d, const int* idim, const int* n, const int* bc_id, const int* nx,
+                        const int* ny, const int* narr, const double* a, const double* a_y,
+                        const double* x, double* y) {
+  run_1d<double>(KDWT, WAV_SYM8, *d, *idim, n, *bc_id, nx, ny, *narr, x, y, *a, 0.0, *a_y, nullptr,
+                 nullptr, nullptr, false);
+}
+void s1s0_1d_sym8_lp_d_(const int* d, const int* idim, const int* n, const int* bc_id, const int* nx,
+                        const int* ny, const int* narr, const double* a, const double* a_y,
+                        const double* x, double* y) {
+  run_1d<double>(KIDWT, WAV_SYM8, *d, *idim, n, *bc_id, nx, ny, *narr, x, y, *a, 0.0, *a_y, nullptr,
+                 nullptr, nullptr, false);
+}
+void s0s1_sym8_lp_d_(const int* d, const int* n, const int* bc, const int* nx, const int* ny,
+                     const int* narr, const double* a, const double* a_y, const double* x, double* y,
+                     double* work) {
+  (void)work;
+  run_md<double>(KDWT, WAV_SYM8, *d, n, bc, nx, ny, *narr, x, y, *a, 0.0, *a_y, nullptr, false);
+}
+void s1s0_sym8_lp_d_(const int* d, const int* n, const int* bc, const int* nx, const int* ny,
+                     const int* narr, const double* a, const double* a_y, const double* x, double* y,
+                     double* work) {
+  (void)work;
+  run_md<double>(KIDWT, WAV_SYM8, *d, n, bc, nx, ny, *narr, x, y, *a, 0.0, *a_y, nullptr, false);
+}
+void s0s0_1d_sym8_md_d_(const int* d, const int* idim, const int* n, const int* bc_id, const int* nx,
+                        const int* ny, const int* narr, const double* a, const double* a_x,
+                        const double* a_y, const double* x, double* y) {
+  run_1d<double>(KCONV, OP_MD, *d, *idim, n, *bc_id, nx, ny, *narr, x, y, *a, *a_x, *a_y, nullptr,
+                 nullptr, nullptr, false);
+}
+void s0s0_1d_sym8_mi_d_(const int* d, const int* idim, const int* n, const int* bc_id, const int* nx,
+                        const int* ny, const int* narr, const double* a, const double* a_x,
+                        const double* a_y, const double* x, double* y) {
+  run_1d<double>(KCONV, OP_IMD, *d, *idim, n, *bc_id, nx, ny, *narr, x, y, *a, *a_x, *a_y, nullptr,
+                 nullptr, nullptr, false);
+}
+void s0s0_1ds_sym8_d2_d_(const int* d, const int* n, const int* bc, const int* nx, const int* ny,
+                         const int* narr, const double* a, const double* a_x, const double* a_y,
+                         const double* x, double* y, double* work) {
+  (void)work;
+  run_1ds<double>(OP_D2, *d, n, bc, nx, ny, *narr, a, *a_x, *a_y, x, y, nullptr, nullptr, nullptr, false);
+}
+void s0s0_1ds_dot_sym8_d2_d_(const int* d, const int* n, const int* bc, const int* nx, const int* ny,
+                             const int* narr, const double* a, const double* a_x, const double* a_y,
+                             const double* x, double* y, double* dot_in, double* work) {
+  (void)work;
+  run_1ds<double>(OP_D2, *d, n, bc, nx, ny, *narr, a, *a_x, *a_y, x, y, dot_in, nullptr, nullptr, false);
+}
+void s0s0_1ds_sym8_d1_d_(const int* d, const int* n, const int* bc, const int* nx, const int* ny,
+                         const int* narr, const double* a, const double* a_x, const double* a_y,
+                         const double* x, double* y, double* work) {
+  (void)work;
+  run_1ds<double>(OP_D1, *d, n, bc, nx, ny, *narr, a, *a_x, *a_y, x, y, nullptr, nullptr, nullptr, false);
+}
+void s0s0_sym8_md_d_(const int* d, const int* n, const int* bc, const int* nx, const int* ny,
+                     const int* narr, const double* a, const double* a_x, const double* a_y,
+                     const double* x, double* y, double* work) {
+  (void)work;
+  run_md<double>(KCONV, OP_MD, *d, n, bc, nx, ny, *narr, x, y, *a, *a_x, *a_y, nullptr, false);
+}
+void s0s0_sym8_mi_d_(const int* d, const int* n, const int* bc, const int* nx, const int* ny,
+                     const int* narr, const double* a, const double* a_x, const double* a_y,
+                     const double* x, double* y, double* work) {
+  (void)work;
+  run_md<double>(KCONV, OP_IMD, *d, n, bc, nx, ny, *narr, x, y, *a, *a_x, *a_y, nullptr, false);
+}
+
+// ---------------------------------------------------------------- stream API
+int lc_conv_1d(int prec, int fil, int d, int idim, const int* n, int bc, const int* nx,
+               const int* ny, int narr, const void* x, void* y, double a, double a_x, double a_y,
+               double* dot_dev, void* stream) {
+  if (prec == 8)
+    return run_1d<double>(KCONV, fil, d, idim, n, bc, nx, ny, narr, (const double*)x, (double*)y, a,
+                          a_x, a_y, nullptr, dot_dev, hs(stream), true);
+  if (prec == 4)
+    return run_1d<float>(KCONV, fil, d, idim, n, bc, nx, ny, narr, (const float*)x, (float*)y, a,
+                         a_x, a_y, nullptr, dot_dev, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_conv_1d: prec %d not in {4, 8}", prec);
+}
+int lc_wavelet_1d(int prec, int wav, int inverse, int d, int idim, const int* n, int bc,
+                  const int* nx, const int* ny, int narr, const void* x, void* y, double a,
+                  double a_y, void* stream) {
+  const int kind = inverse ? KIDWT : KDWT;
+  if (prec == 8)
+    return run_1d<double>(kind, wav, d, idim, n, bc, nx, ny, narr, (const double*)x, (double*)y, a,
+                          0.0, a_y, nullptr, nullptr, hs(stream), true);
+  if (prec == 4)
+    return run_1d<float>(kind, wav, d, idim, n, bc, nx, ny, narr, (const float*)x, (float*)y, a,
+                         0.0, a_y, nullptr, nullptr, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_wavelet_1d: prec %d not in {4, 8}", prec);
+}
+int lc_conv_md(int prec, int fil, int d, const int* n, const int* bc, const int* nx, const int* ny,
+               int narr, const void* x, void* y, double a, double a_x, double a_y, void* stream) {
+  if (prec == 8)
+    return run_md<double>(KCONV, fil, d, n, bc, nx, ny, narr, (const double*)x, (double*)y, a, a_x,
+                          a_y, hs(stream), true);
+  if (prec == 4)
+    return run_md<float>(KCONV, fil, d, n, bc, nx, ny, narr, (const float*)x, (float*)y, a, a_x,
+                         a_y, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_conv_md: prec %d not in {4, 8}", prec);
+}
+int lc_conv_1ds(int prec, int fil, int d, const int* n, const int* bc, const int* nx, const int* ny,
+                int narr, const double* a, double a_x, double a_y, const void* x, void* y,
+                double* dot_dev, void* stream) {
+  if (prec == 8)
+    return run_1ds<double>(fil, d, n, bc, nx, ny, narr, a, a_x, a_y, (const double*)x, (double*)y,
+                           nullptr, dot_dev, hs(stream), true);
+  if (prec == 4)
+    return run_1ds<float>(fil, d, n, bc, nx, ny, narr, a, a_x, a_y, (const float*)x, (float*)y,
+                          nullptr, dot_dev, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_conv_1ds: prec %d not in {4, 8}", prec);
+}
+int lc_wavelet_md(int prec, int wav, int inverse, int d, const int* n, const int* bc,
+                  const int* nx, const int* ny, int narr, const void* x, void* y, double a,
+                  double a_y, void* stream) {
+  const int kind = inverse ? KIDWT : KDWT;
+  if (prec == 8)
+    return run_md<double>(kind, wav, d, n, bc, nx, ny, narr, (const double*)x, (double*)y, a, 0.0,
+                          a_y, hs(stream), true);
+  if (prec == 4)
+    return run_md<float>(kind, wav, d, n, bc, nx, ny, narr, (const float*)x, (float*)y, a, 0.0, a_y,
+                         hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_wavelet_md: prec %d not in {4, 8}", prec);
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// lc_kernels.hpp -- launch interfaces of the libconv HIP kernels (internal).
+#pragma once
+
+#include "lc_common.hpp"
+
+namespace lc {
+
+enum OpId : int { OP_MD = 0, OP_IMD = 1, OP_D2 = 2, OP_D1 = 3 };
+enum WavId : int { WAV_SYM8 = 0, WAV_SYMICOMB8 = 1 };
+
+// One 1D sweep over narr arrays seen as (L, line, R).
+template <typename R>
+struct SweepArgs {
+  const R* x;
+  R* y;
+  long L;         // stride between steps of a line (elements)
+  long nr;        // lines per array along the slower dims
+  long ncols;     // L * R * narr
+  long line_in;   // distance between consecutive r in x (= L * IN * nx)
+  long line_out;  // distance between consecutive r in y (= L * OUT * ny)
+  long nti, nto;  // distance between consecutive arrays
+  int narr;
+  int n, nx, ny;  // logical length and extents along the line (steps)
+  int ox, oy;     // logical index of the first stored input / output step
+  int i_first;    // first output step (-up for grow, else 0)
+  int nout;       // number of output steps
+  int chunk;      // outputs per lane chunk (strided kernel; set by launcher)
+  R a, a_x, a_y;
+  int flags;      // F_A | F_AX | F_AY | F_DOT
+  double* dot_part;  // per-block FP64 partials of <x, a f(x)> (or null)
+};
+
+template <typename R>
+hipError_t sweep(int kind, int id, int bc, const SweepArgs<R>& p, hipStream_t st, double* dot_out);
+template <typename R>
+long sweep_partials_needed(const SweepArgs<R>& p, int W);
+hipError_t sum_partial_sets(const double* part, long nparts, double* out, int nsets, long stride,
+                            hipStream_t st);
+
+// Filter geometry for the host side.
+struct OpGeom {
+  int lo, up, w;  // step offsets and window (pairs for wavelets)
+};
+OpGeom conv_geom(int id);
+OpGeom wavelet_geom(int id, int recompose, int bc);
+
+// ------------------------------------------------------------ 3D operators
+// Box of one 3D array: logical extent n[3], BC per dim, storage extents
+// nx[3] (input) and ny[3] (output).
+struct Box3 {
+  int n[3], bc[3], nx[3], ny[3];
+};
+
+// separable product y = a * F2 F1 F0 x (+ a_y y), fused single pass
+template <typename R>
+hipError_t sep3d(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                 double* scratch, size_t scratch_bytes);
+// kinetic-style sum y = sum_d a_d F_d x + a_x x + a_y y with per-dim
+// energies dot[d] = <x, a_d F_d x> (of the last array) when dot != null
+template <typename R>
+hipError_t sum3d(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
+                 double* dot, hipStream_t st, double* scratch, size_t scratch_bytes);
+template <typename R>
+size_t sum3d_scratch_bytes(const Box3& b, int narr);
+template <typename R>
+size_t sep3d_scratch_bytes(const Box3& b, int narr);
+
+}  // namespace lc

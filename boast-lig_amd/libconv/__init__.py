@@ -1,0 +1,327 @@
+"""Host-side mirror of the reference's kernel interface over libconv_amd.so.
+
+The reference drives each generated libconv procedure through BOAST's
+`CKernel#run(*args)`, passing the procedure's arguments in order and reading
+back `{:duration, :reference_return => {:dot_in}}` (e.g.
+BigDFT/MagicFilter1d.rb:55, BigDFT/Kinetic1d-nb.ipynb cells 15-18).  This
+module gives the same shape in Python:
+
+    k = libconv.MagicFilter1d()              # d_s0s0_1d_sym8_md
+    stats = k.run(3, 0, n, libconv.BC.PERIODIC, n, n, 1, x, y, 1.0, 0.0, 0.0)
+    stats['duration'], stats['reference_return']
+
+Arrays may be numpy arrays (host memory; the library stages them) or torch
+tensors on the GPU (device memory; run in place).  Every call goes through
+the C ABI of include/libconv.h; there is no other compute path, and a
+missing library raises at import.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(HERE), 'libconv_amd.so')
+
+
+class BC:
+    """Boundary conditions (BigDFT/GenericConvolution2.rb:684-740)."""
+    PERIODIC = 0
+    GROW = 1
+    SHRINK = -1
+
+
+class LibconvError(RuntimeError):
+    pass
+
+
+_lib = None
+_I = ctypes.c_int
+_IP = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+_D = ctypes.c_double
+_F = ctypes.c_float
+_DP = ctypes.POINTER(ctypes.c_double)
+_FP = ctypes.POINTER(ctypes.c_float)
+
+
+def lib():
+    """The loaded libconv_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise LibconvError('libconv_amd.so not built (run make -C boast-lig_amd)')
+        try:  # load torch's HIP runtime first so the process holds a single one
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIBPATH)
+        for p, R in (('d', _D), ('s', _F)):
+            for f in ('md', 'imd', 'd2', 'd1'):
+                fn = getattr(L, '%s_s0s0_1d_sym8_%s' % (p, f))
+                fn.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, R, R, R]
+                fn.restype = None
+            for w in ('s0s1', 's1s0'):
+                for fam in ('sym8', 'symicomb8'):
+                    fn = getattr(L, '%s_%s_1d_%s' % (p, w, fam))
+                    fn.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, R, R]
+                    fn.restype = None
+            for f in ('md', 'imd'):
+                fn = getattr(L, '%s_sym8_%s_ld_narr' % (p, f))
+                fn.argtypes = [_I, _IP, _IP, _IP, _IP, _I, _VP, _VP, _VP, _VP]
+                fn.restype = None
+            fn = getattr(L, '%s_sym8_d2' % p)
+            fn.argtypes = [_I, _IP, _IP, _VP, _VP, _VP, _VP, _VP, R, R]
+            fn.restype = None
+            for w in ('dwt', 'idwt'):
+                fn = getattr(L, '%s_%s_sym8' % (p, w))
+                fn.argtypes = [_I, _IP, _IP, _VP, _VP, _VP, _VP]
+                fn.restype = None
+        L.d_s0s0_1d_sym8_d2_dotin.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _D, _D, _D, _DP]
+        L.s_s0s0_1d_sym8_d2_dotin.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _F, _F, _F, _FP]
+        L.s_s0s0_1d_sym8_d2_dotin64.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _F, _F, _F, _DP]
+        L.d_sym8_d2_dotin.argtypes = [_I, _IP, _IP, _VP, _VP, _VP, _VP, _VP, _D, _D, _DP]
+        L.s_sym8_d2_dotin64.argtypes = [_I, _IP, _IP, _VP, _VP, _VP, _VP, _VP, _F, _F, _DP]
+        for n in ('d_s0s0_1d_sym8_d2_dotin', 's_s0s0_1d_sym8_d2_dotin', 's_s0s0_1d_sym8_d2_dotin64',
+                  'd_sym8_d2_dotin', 's_sym8_d2_dotin64'):
+            getattr(L, n).restype = None
+        L.d_s0s0_1d_sym8_md_cost.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _D, _D, _D, _IP]
+        L.d_s0s1_1d_sym8_cost.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _D, _D, _IP]
+        L.d_s1s0_1d_sym8_cost.argtypes = [_I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _D, _D, _IP]
+        L.lc_last_error.restype = _I
+        L.lc_last_error_message.restype = ctypes.c_char_p
+        L.lc_version.restype = ctypes.c_char_p
+        L.lc_extent.argtypes = [_I, _I, _I, _I]
+        L.lc_extent.restype = _I
+        L.lc_conv_1d.argtypes = [_I, _I, _I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _D, _D, _D, _VP, _VP]
+        L.lc_wavelet_1d.argtypes = [_I, _I, _I, _I, _I, _IP, _I, _IP, _IP, _I, _VP, _VP, _D, _D, _VP]
+        L.lc_conv_md.argtypes = [_I, _I, _I, _IP, _IP, _IP, _IP, _I, _VP, _VP, _D, _D, _D, _VP]
+        L.lc_conv_1ds.argtypes = [_I, _I, _I, _IP, _IP, _IP, _IP, _I, _DP, _D, _D, _VP, _VP, _VP, _VP]
+        L.lc_wavelet_md.argtypes = [_I, _I, _I, _I, _IP, _IP, _IP, _IP, _I, _VP, _VP, _D, _D, _VP]
+        for n in ('lc_conv_1d', 'lc_wavelet_1d', 'lc_conv_md', 'lc_conv_1ds', 'lc_wavelet_md'):
+            getattr(L, n).restype = _I
+        _lib = L
+    return _lib
+
+
+def version():
+    return lib().lc_version().decode()
+
+
+def check():
+    """Raise LibconvError if the last call of this thread recorded an error."""
+    code = lib().lc_last_error()
+    if code:
+        raise LibconvError('libconv error %d: %s' % (code, lib().lc_last_error_message().decode()))
+
+
+# ------------------------------------------------------------------ arguments
+def _ints(v):
+    if isinstance(v, (int, np.integer)):
+        v = [int(v)]
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.int32))
+    return a
+
+
+def _ptr(a):
+    """Address of a numpy array or a torch tensor (device or host)."""
+    if a is None:
+        return None
+    if hasattr(a, 'data_ptr'):
+        return ctypes.c_void_p(a.data_ptr())
+    if isinstance(a, np.ndarray):
+        return ctypes.c_void_p(a.ctypes.data)
+    raise TypeError('array argument must be a numpy array or a torch tensor')
+
+
+def _is_f32(a):
+    if hasattr(a, 'dtype') and str(a.dtype) in ('float32', 'torch.float32'):
+        return True
+    return False
+
+
+class LibconvKernel:
+    """One libconv procedure with the reference's `run` calling shape."""
+
+    def __init__(self, name, fp32=False):
+        self.name = ('s_' if fp32 else 'd_') + name
+        self.fp32 = fp32
+        self.fn = getattr(lib(), self.name)
+
+    def _marshal(self, args):
+        keep, out = [], []
+        for a in args:
+            if isinstance(a, (list, tuple)) and not all(isinstance(v, (int, np.integer)) for v in a):
+                arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32 if self.fp32 else np.float64))
+                keep.append(arr)
+                out.append(ctypes.c_void_p(arr.ctypes.data))
+            elif isinstance(a, (list, tuple)) or (isinstance(a, np.ndarray) and a.dtype.kind == 'i'):
+                arr = _ints(a)
+                keep.append(arr)
+                out.append(arr.ctypes.data_as(_IP))
+            elif isinstance(a, np.ndarray) or hasattr(a, 'data_ptr'):
+                keep.append(a)
+                out.append(_ptr(a))
+            else:
+                out.append(a)
+        return keep, out
+
+    def run(self, *args, **kw):
+        keep, cargs = self._marshal(args)
+        dot = None
+        if kw.get('dot'):
+            dot = (_F if (self.fp32 and not kw.get('dot64')) else _D)(0.0)
+            cargs.append(ctypes.byref(dot))
+        sync = kw.get('sync')
+        t0 = time.perf_counter()
+        self.fn(*cargs)
+        if sync:
+            sync()
+        dt = time.perf_counter() - t0
+        check()
+        stats = {'duration': dt}
+        if dot is not None:
+            stats['reference_return'] = {'dot_in': dot.value}
+        return stats
+
+
+# ------------------------------------------------------------------ front-ends
+# (BigDFT/LibconvBase.rb:226-262 generate_*, MagicFilter1d.rb, Kinetic1d.rb,
+#  Wavelet1d2.rb, MagicFilter.rb, Kinetic.rb, Wavelet2.rb)
+class _DotKernel(LibconvKernel):
+    def run(self, *args, **kw):
+        # the generated procedure takes dot_in as its last argument; the value
+        # passed in is ignored and the result comes back in reference_return
+        args = args[:-1] if len(args) == self.nargs else args
+        kw['dot'] = True
+        return LibconvKernel.run(self, *args, **kw)
+
+
+def MagicFilter1d(fp32=False):
+    """d_s0s0_1d_sym8_md: (d, idim, n, bc, nx, ny, narr, x, y, a, a_x, a_y)."""
+    return LibconvKernel('s0s0_1d_sym8_md', fp32)
+
+
+def InverseMagicFilter1d(fp32=False):
+    return LibconvKernel('s0s0_1d_sym8_imd', fp32)
+
+
+def Kinetic1d(fp32=False, dot64=False):
+    """d_s0s0_1d_sym8_d2_dotin: (..., a, a_x, a_y, dot_in) -> reference_return."""
+    name = 's0s0_1d_sym8_d2_dotin64' if (fp32 and dot64) else 's0s0_1d_sym8_d2_dotin'
+    k = _DotKernel(name, fp32)
+    if fp32 and dot64:
+        k.name = 's_' + name
+        k.fn = getattr(lib(), k.name)
+        k.fp32 = False  # the energy comes back in FP64
+    k.nargs = 13
+    return k
+
+
+def Wavelet1d(direction, fp32=False, combined=False):
+    """d_s0s1_1d_sym8 (direction 'decompose') / d_s1s0_1d_sym8 ('recompose');
+    combined=True gives the symicomb8 operators rtos1 / s1tor."""
+    base = 's0s1' if direction == 'decompose' else 's1s0'
+    return LibconvKernel('%s_1d_%s' % (base, 'symicomb8' if combined else 'sym8'), fp32)
+
+
+def MagicFilter(fp32=False, inverse=False):
+    """d_sym8_md_ld_narr: (d, n, bc, nx, ny, narr, x, y, w1, w2)."""
+    return LibconvKernel('sym8_%s_ld_narr' % ('imd' if inverse else 'md'), fp32)
+
+
+def Kinetic(fp32=False, dot=False):
+    """d_sym8_d2: (d, n, bc, x, y, w1, w2, a, a_x, a_y); with dot=True the
+    per-direction energies come back in reference_return['dot_in']."""
+    if not dot:
+        return LibconvKernel('sym8_d2', fp32)
+    return _KineticDot(fp32)
+
+
+class _KineticDot(LibconvKernel):
+    def __init__(self, fp32):
+        self.name = 's_sym8_d2_dotin64' if fp32 else 'd_sym8_d2_dotin'
+        self.fp32 = fp32
+        self.fn = getattr(lib(), self.name)
+
+    def run(self, d, n, bc, x, y, w1, w2, a, a_x, a_y, **kw):
+        dots = np.zeros(int(d))
+        av = np.ascontiguousarray(np.asarray(a, dtype=np.float32 if self.fp32 else np.float64))
+        keep, cargs = self._marshal([d, n, bc, x, y, w1, w2])
+        t0 = time.perf_counter()
+        self.fn(*cargs, ctypes.c_void_p(av.ctypes.data), a_x, a_y, dots.ctypes.data_as(_DP))
+        dt = time.perf_counter() - t0
+        check()
+        return {'duration': dt, 'reference_return': {'dot_in': dots}}
+
+
+def Wavelet(direction, fp32=False):
+    """d_dwt_sym8 / d_idwt_sym8: (d, n, bc, x, y, w1, w2)."""
+    return LibconvKernel('%s_sym8' % ('dwt' if direction == 'decompose' else 'idwt'), fp32)
+
+
+# ------------------------------------------------------------------ stream API
+def _stream(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, 'cuda_stream'):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+def _prec(t):
+    return 4 if _is_f32(t) else 8
+
+
+def _rc(rc):
+    if rc:
+        check()
+        raise LibconvError('libconv returned %d' % rc)
+
+
+def conv_1d(fil, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0, dot=None, stream=None):
+    """Device-pointer sweep (lc_conv_1d); dot: a float64 device tensor or None."""
+    n_, nx_, ny_ = _ints(n), _ints(nx), _ints(ny)
+    _rc(lib().lc_conv_1d(_prec(x), fil, len(n_), idim, n_.ctypes.data_as(_IP), bc,
+                         nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr, _ptr(x), _ptr(y),
+                         a, a_x, a_y, _ptr(dot), _stream(stream)))
+
+
+def wavelet_1d(wav, inverse, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0, stream=None):
+    n_, nx_, ny_ = _ints(n), _ints(nx), _ints(ny)
+    _rc(lib().lc_wavelet_1d(_prec(x), wav, int(bool(inverse)), len(n_), idim, n_.ctypes.data_as(_IP),
+                            bc, nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr, _ptr(x),
+                            _ptr(y), a, a_y, _stream(stream)))
+
+
+def conv_md(fil, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0, stream=None):
+    n_, b_, nx_, ny_ = _ints(n), _ints(bc), _ints(nx), _ints(ny)
+    _rc(lib().lc_conv_md(_prec(x), fil, len(n_), n_.ctypes.data_as(_IP), b_.ctypes.data_as(_IP),
+                         nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr, _ptr(x), _ptr(y),
+                         a, a_x, a_y, _stream(stream)))
+
+
+def conv_1ds(fil, n, bc, nx, ny, narr, a, a_x, a_y, x, y, dot=None, stream=None):
+    n_, b_, nx_, ny_ = _ints(n), _ints(bc), _ints(nx), _ints(ny)
+    av = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    _rc(lib().lc_conv_1ds(_prec(x), fil, len(n_), n_.ctypes.data_as(_IP), b_.ctypes.data_as(_IP),
+                          nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr,
+                          av.ctypes.data_as(_DP), a_x, a_y, _ptr(x), _ptr(y), _ptr(dot),
+                          _stream(stream)))
+
+
+def wavelet_md(wav, inverse, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0, stream=None):
+    n_, b_, nx_, ny_ = _ints(n), _ints(bc), _ints(nx), _ints(ny)
+    _rc(lib().lc_wavelet_md(_prec(x), wav, int(bool(inverse)), len(n_), n_.ctypes.data_as(_IP),
+                            b_.ctypes.data_as(_IP), nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP),
+                            narr, _ptr(x), _ptr(y), a, a_y, _stream(stream)))
+
+
+FIL_MD, FIL_IMD, FIL_D2, FIL_D1 = 0, 1, 2, 3
+WAV_SYM8, WAV_SYMICOMB8 = 0, 1
+
+
+def extent(op, bc, n, side):
+    """Input (side 0) / output (side 1) extent along a dim (lc_extent)."""
+    return lib().lc_extent(op, bc, n, side)

@@ -1,0 +1,234 @@
+/*
+ * libconv.h -- C ABI of the MI355X (gfx950) libconv library.
+ *
+ * Drop-in replacement for the procedures that the BOAST generator of the
+ * reference emits for BigDFT's separable convolutions
+ * (BigDFT/GenericConvolution2.rb), plus the spellings of the intended
+ * public interface (BigDFT/libconv.f90:303-456) and a stream-aware API for
+ * device-resident data.  Arrays are Fortran-ordered (dim 0 fastest).
+ *
+ * Conventions of the drop-in names (BOAST C flavour, GenericConvolution2.rb
+ * :1444-1766 and :1768-2245):
+ *   - int and real scalars by value, arrays by pointer, out-scalars by
+ *     pointer; d_ = FP64, s_ = FP32 storage and arithmetic;
+ *   - n[] logical sizes (pairs for wavelets), nx[]/ny[] the input/output
+ *     extents along each dim (the "ld" leading dims; the non-processed dims
+ *     use nx[] for both arrays), narr arrays of the same shape back to back;
+ *   - bc: 0 periodic, 1 free grow (output extended by the filter length),
+ *     -1 free shrink (input extended);
+ *   - a == 1 skips the scaling, a_x == 0 the a_x * x term and a_y == 0 the
+ *     read of y, exactly as the generated dispatcher selects its variants;
+ *   - pointers may be host or device memory (detected per call); results
+ *     are complete on return; the functions are reentrant per host thread;
+ *   - the reference procedures return nothing and check nothing: these
+ *     validate their arguments, leave y untouched on error and record the
+ *     error for lc_last_error() (LIBCONV_ABORT_ON_ERROR=1 aborts instead).
+ * dot_in follows the generated dispatcher: <x, a fil(x)> of the last array
+ * of the narr batch, accumulated in FP64 on the GPU.
+ *
+ * Free BCs with n below the filter width: the generated sweeps read out of
+ * bounds there; this library treats x as zero outside its box (both sides
+ * clamped) and wraps periodic indices with a true modulo.
+ */
+#ifndef LIBCONV_H
+#define LIBCONV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+enum {
+  LC_OK = 0,
+  LC_ERR_ARG = 1,      /* invalid dims / extents / bc / ids */
+  LC_ERR_UNSUPPORTED = 2,
+  LC_ERR_DEVICE = 3,   /* HIP runtime error (message holds the HIP text) */
+  LC_ERR_NOMEM = 4
+};
+int lc_last_error(void);                 /* last error of this host thread, then cleared */
+const char *lc_last_error_message(void); /* text of the last error of this host thread */
+
+/* ------------------------------------------------- libconv 1D sweeps (C)
+ * y := a * fil |X_idim| x + a_x * x + a_y * y     along dim idim
+ * Replaces the generated dispatchers d_s0s0_1d_<filter>
+ * (GenericConvolutionOperator1d#procedure, GenericConvolution2.rb:1560-1766;
+ * built by MagicFilter1d, BigDFT/LibconvBase.rb:10-30, 238-248).        */
+void d_s0s0_1d_sym8_md(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                       int narr, const double *x, double *y, double a, double a_x, double a_y);
+void s_s0s0_1d_sym8_md(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                       int narr, const float *x, float *y, float a, float a_x, float a_y);
+void d_s0s0_1d_sym8_imd(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                        int narr, const double *x, double *y, double a, double a_x, double a_y);
+void s_s0s0_1d_sym8_imd(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                        int narr, const float *x, float *y, float a, float a_x, float a_y);
+void d_s0s0_1d_sym8_d2(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                       int narr, const double *x, double *y, double a, double a_x, double a_y);
+void s_s0s0_1d_sym8_d2(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                       int narr, const float *x, float *y, float a, float a_x, float a_y);
+void d_s0s0_1d_sym8_d1(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                       int narr, const double *x, double *y, double a, double a_x, double a_y);
+void s_s0s0_1d_sym8_d1(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                       int narr, const float *x, float *y, float a, float a_x, float a_y);
+/* with dot_in = <x, a d2(x)> (Kinetic1d with :dot_in, BigDFT/Kinetic1d-nb.ipynb cell 1) */
+void d_s0s0_1d_sym8_d2_dotin(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                             int narr, const double *x, double *y, double a, double a_x, double a_y,
+                             double *dot_in);
+void s_s0s0_1d_sym8_d2_dotin(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                             int narr, const float *x, float *y, float a, float a_x, float a_y,
+                             float *dot_in);
+/* FP32 storage with the FP64 energy (no counterpart in the reference) */
+void s_s0s0_1d_sym8_d2_dotin64(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                               int narr, const float *x, float *y, float a, float a_x, float a_y,
+                               double *dot_in);
+
+/* ------------------------------------------------ libconv 1D wavelets (C)
+ * s0s1: y := a * dwt |X_idim| x + a_y * y   (analysis, output [L | H])
+ * s1s0: y := a * idwt |X_idim| x + a_y * y  (synthesis)
+ * Replace d_s0s1_1d_sym8 / d_s1s0_1d_sym8 (Wavelet1d, BigDFT/LibconvBase.rb
+ * :31-49, 226-236; generated listing BigDFT/Wavelet1d-nb.ipynb cell 4) and
+ * the combined operators rtos1 = d_s0s1_1d_symicomb8 (DWT of the inverse
+ * magic filter) and s1tor = d_s1s0_1d_symicomb8 (magic filter of the IDWT)
+ * (BigDFT/LibconvBase.rb:250-262).                                         */
+void d_s0s1_1d_sym8(int d, int idim, const int *n, int bc, const int *nx, const int *ny, int narr,
+                    const double *x, double *y, double a, double a_y);
+void s_s0s1_1d_sym8(int d, int idim, const int *n, int bc, const int *nx, const int *ny, int narr,
+                    const float *x, float *y, float a, float a_y);
+void d_s1s0_1d_sym8(int d, int idim, const int *n, int bc, const int *nx, const int *ny, int narr,
+                    const double *x, double *y, double a, double a_y);
+void s_s1s0_1d_sym8(int d, int idim, const int *n, int bc, const int *nx, const int *ny, int narr,
+                    const float *x, float *y, float a, float a_y);
+void d_s0s1_1d_symicomb8(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                         int narr, const double *x, double *y, double a, double a_y);
+void s_s0s1_1d_symicomb8(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                         int narr, const float *x, float *y, float a, float a_y);
+void d_s1s0_1d_symicomb8(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                         int narr, const double *x, double *y, double a, double a_y);
+void s_s1s0_1d_symicomb8(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                         int narr, const float *x, float *y, float a, float a_y);
+
+/* ------------------------------------------------------- cost companions
+ * Flop count of a call (the generated *_cost procedures,
+ * GenericConvolution2.rb:1269-1276, 1561-1568): narr * n[idim] * ndat *
+ * (2 L for a filter of L taps, 64 per pair for sym8 wavelets).            */
+void d_s0s0_1d_sym8_md_cost(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                            int narr, const double *x, double *y, double a, double a_x, double a_y,
+                            int *cost);
+void d_s0s0_1d_sym8_d2_dotin_cost(int d, int idim, const int *n, int bc, const int *nx,
+                                  const int *ny, int narr, const double *x, double *y, double a,
+                                  double a_x, double a_y, double *dot_in, int *cost);
+void d_s0s1_1d_sym8_cost(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                         int narr, const double *x, double *y, double a, double a_y, int *cost);
+void d_s1s0_1d_sym8_cost(int d, int idim, const int *n, int bc, const int *nx, const int *ny,
+                         int narr, const double *x, double *y, double a, double a_y, int *cost);
+
+/* ------------------------------------------------- multi-dimensional (C)
+ * Replace the GenericConvolutionOperator procedures (GenericConvolution2.rb
+ * :2131-2245) built by MagicFilter (BigDFT/MagicFilter.rb:46-70: transpose 1,
+ * work, ld, narr), Kinetic (BigDFT/Kinetic.rb:1068-1090: transpose 0, a,
+ * a_x, a_y, work, kinetic, zero_out_work) and Wavelet (BigDFT/Wavelet2.rb:
+ * 2-21: transpose 1, work, wavelet).  The work arrays w1/w2 are accepted
+ * for ABI compatibility; the library keeps its own device scratch.       */
+void d_sym8_md_ld_narr(int d, const int *n, const int *bc, const int *nx, const int *ny, int narr,
+                       const double *x, double *y, double *w1, double *w2);
+void s_sym8_md_ld_narr(int d, const int *n, const int *bc, const int *nx, const int *ny, int narr,
+                       const float *x, float *y, float *w1, float *w2);
+void d_sym8_imd_ld_narr(int d, const int *n, const int *bc, const int *nx, const int *ny, int narr,
+                        const double *x, double *y, double *w1, double *w2);
+void s_sym8_imd_ld_narr(int d, const int *n, const int *bc, const int *nx, const int *ny, int narr,
+                        const float *x, float *y, float *w1, float *w2);
+/* y = sum_i a[i] d2_i(x) + a_x x + a_y y  (x extents per BC from n) */
+void d_sym8_d2(int d, const int *n, const int *bc, const double *x, double *y, double *w1,
+               double *w2, const double *a, double a_x, double a_y);
+void s_sym8_d2(int d, const int *n, const int *bc, const float *x, float *y, float *w1, float *w2,
+               const float *a, float a_x, float a_y);
+/* same with the per-direction energies dot_in[i] = <x, a[i] d2_i(x)> */
+void d_sym8_d2_dotin(int d, const int *n, const int *bc, const double *x, double *y, double *w1,
+                     double *w2, const double *a, double a_x, double a_y, double *dot_in);
+void s_sym8_d2_dotin64(int d, const int *n, const int *bc, const float *x, float *y, float *w1,
+                       float *w2, const float *a, float a_x, float a_y, double *dot_in);
+void d_dwt_sym8(int d, const int *n, const int *bc, const double *x, double *y, double *w1,
+                double *w2);
+void s_dwt_sym8(int d, const int *n, const int *bc, const float *x, float *y, float *w1, float *w2);
+void d_idwt_sym8(int d, const int *n, const int *bc, const double *x, double *y, double *w1,
+                 double *w2);
+void s_idwt_sym8(int d, const int *n, const int *bc, const float *x, float *y, float *w1,
+                 float *w2);
+
+/* --------------------------------------- libconv.f90 spellings (Fortran)
+ * BigDFT/libconv.f90:303-456 (alpha release names), every argument by
+ * reference, lower case with a trailing underscore.  s1s0_1d_sym8_lp_d
+ * takes idim (the spec omits it, BigDFT/libconv.f90:320-332), bc_id is a
+ * scalar for the 1D names.                                                 */
+void s0s1_1d_sym8_lp_d_(const int *d, const int *idim, const int *n, const int *bc_id,
+                        const int *nx, const int *ny, const int *narr, const double *a,
+                        const double *a_y, const double *x, double *y);
+void s1s0_1d_sym8_lp_d_(const int *d, const int *idim, const int *n, const int *bc_id,
+                        const int *nx, const int *ny, const int *narr, const double *a,
+                        const double *a_y, const double *x, double *y);
+void s0s1_sym8_lp_d_(const int *d, const int *n, const int *bc, const int *nx, const int *ny,
+                     const int *narr, const double *a, const double *a_y, const double *x,
+                     double *y, double *work);
+void s1s0_sym8_lp_d_(const int *d, const int *n, const int *bc, const int *nx, const int *ny,
+                     const int *narr, const double *a, const double *a_y, const double *x,
+                     double *y, double *work);
+void s0s0_1d_sym8_md_d_(const int *d, const int *idim, const int *n, const int *bc_id,
+                        const int *nx, const int *ny, const int *narr, const double *a,
+                        const double *a_x, const double *a_y, const double *x, double *y);
+void s0s0_1d_sym8_mi_d_(const int *d, const int *idim, const int *n, const int *bc_id,
+                        const int *nx, const int *ny, const int *narr, const double *a,
+                        const double *a_x, const double *a_y, const double *x, double *y);
+void s0s0_1ds_sym8_d2_d_(const int *d, const int *n, const int *bc, const int *nx, const int *ny,
+                         const int *narr, const double *a, const double *a_x, const double *a_y,
+                         const double *x, double *y, double *work);
+void s0s0_1ds_dot_sym8_d2_d_(const int *d, const int *n, const int *bc, const int *nx,
+                             const int *ny, const int *narr, const double *a, const double *a_x,
+                             const double *a_y, const double *x, double *y, double *dot_in,
+                             double *work);
+void s0s0_1ds_sym8_d1_d_(const int *d, const int *n, const int *bc, const int *nx, const int *ny,
+                         const int *narr, const double *a, const double *a_x, const double *a_y,
+                         const double *x, double *y, double *work);
+void s0s0_sym8_md_d_(const int *d, const int *n, const int *bc, const int *nx, const int *ny,
+                     const int *narr, const double *a, const double *a_x, const double *a_y,
+                     const double *x, double *y, double *work);
+void s0s0_sym8_mi_d_(const int *d, const int *n, const int *bc, const int *nx, const int *ny,
+                     const int *narr, const double *a, const double *a_x, const double *a_y,
+                     const double *x, double *y, double *work);
+
+/* ------------------------------------------- stream API (device memory)
+ * New code: device pointers only, asynchronous on `stream` (a hipStream_t,
+ * NULL = the default stream), int return (LC_OK or an error code).  prec is
+ * 8 (FP64) or 4 (FP32 storage and arithmetic); scalars are given in double
+ * and rounded to the storage precision; dot outputs are device pointers to
+ * FP64 (d values for lc_conv_1ds), written asynchronously.
+ * fil: 0 magic filter (sym8_md), 1 inverse magic filter (sym8_imd),
+ *      2 second derivative (sym8_d2), 3 first derivative (sym8_d1);
+ * wav: 0 sym8, 1 symicomb8 (magic filter combined with the wavelet).       */
+enum { LC_FIL_MD = 0, LC_FIL_IMD = 1, LC_FIL_D2 = 2, LC_FIL_D1 = 3 };
+enum { LC_WAV_SYM8 = 0, LC_WAV_SYMICOMB8 = 1 };
+int lc_conv_1d(int prec, int fil, int d, int idim, const int *n, int bc, const int *nx,
+               const int *ny, int narr, const void *x, void *y, double a, double a_x, double a_y,
+               double *dot_dev, void *stream);
+int lc_wavelet_1d(int prec, int wav, int inverse, int d, int idim, const int *n, int bc,
+                  const int *nx, const int *ny, int narr, const void *x, void *y, double a,
+                  double a_y, void *stream);
+/* separable product over all d dims: y = a fil_{d-1}..fil_0 x + a_x x + a_y y */
+int lc_conv_md(int prec, int fil, int d, const int *n, const int *bc, const int *nx,
+               const int *ny, int narr, const void *x, void *y, double a, double a_x, double a_y,
+               void *stream);
+/* sum over dims: y = sum_i a[i] fil_i x + a_x x + a_y y; a[] on the host */
+int lc_conv_1ds(int prec, int fil, int d, const int *n, const int *bc, const int *nx,
+                const int *ny, int narr, const double *a, double a_x, double a_y, const void *x,
+                void *y, double *dot_dev, void *stream);
+int lc_wavelet_md(int prec, int wav, int inverse, int d, const int *n, const int *bc,
+                  const int *nx, const int *ny, int narr, const void *x, void *y, double a,
+                  double a_y, void *stream);
+/* extent of the input (side 0) or output (side 1) along a dim for an
+ * operator: op 0..3 = fil ids, 10 + wav = analysis, 20 + wav = synthesis */
+int lc_extent(int op, int bc, int n, int side);
+/* library version string */
+const char *lc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIBCONV_H */

@@ -1,0 +1,87 @@
+"""The C ABI library: builds, loads, exports every symbol include/libconv.h
+declares, and rejects invalid arguments before touching a device (CPU only:
+no compute call is made here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'libconv.h')
+LIB = os.path.join(ROOT, 'boast-lig_amd', 'libconv_amd.so')
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    names = re.findall(r'^\s*(?:const\s+)?(?:void|int|char\s*\*|const char\s*\*)\s*\**\s*(\w+)\s*\(', text, re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope='module')
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail('libconv_amd.so missing: run __graft_entry__.build()')
+    import libconv
+    return libconv.lib()
+
+
+def test_header_declares_the_drop_in_names():
+    names = declared()
+    for must in ('d_s0s0_1d_sym8_md', 'd_s0s0_1d_sym8_d2_dotin', 'd_s0s1_1d_sym8', 'd_s1s0_1d_sym8',
+                 'd_sym8_md_ld_narr', 'd_sym8_d2', 'd_dwt_sym8', 'd_idwt_sym8', 's0s0_1d_sym8_md_d_',
+                 's0s0_1ds_dot_sym8_d2_d_', 'lc_conv_1d', 'lc_last_error'):
+        assert must in names, must
+
+
+def test_library_exports_every_declared_symbol(lib):
+    out = subprocess.run(['nm', '-D', '--defined-only', LIB], capture_output=True, text=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if ' T ' in l)
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+
+
+def test_version_and_extents(lib):
+    import libconv
+    assert 'gfx950' in libconv.version()
+    # MF grow +15, kinetic shrink +28, sym8 grow +7 pairs, symicomb8 +15
+    assert libconv.extent(0, 1, 100, 1) == 115 and libconv.extent(0, 1, 100, 0) == 100
+    assert libconv.extent(2, -1, 100, 0) == 128 and libconv.extent(2, -1, 100, 1) == 100
+    assert libconv.extent(10, 1, 50, 1) == 57 and libconv.extent(21, -1, 50, 0) == 65
+    assert libconv.extent(0, 5, 10, 0) == -1
+
+
+@pytest.mark.parametrize('case', ['d0', 'idim', 'bc', 'nx_small', 'ny_grow', 'narr'])
+def test_invalid_arguments_are_reported(lib, case):
+    import libconv
+    n = np.array([8, 8, 8], np.int32)
+    nx, ny = n.copy(), n.copy()
+    d, idim, bc, narr = 3, 0, 0, 1
+    if case == 'd0':
+        d = 0
+    elif case == 'idim':
+        idim = 3
+    elif case == 'bc':
+        bc = 7
+    elif case == 'nx_small':
+        bc = -1  # shrink needs nx >= n + 15
+    elif case == 'ny_grow':
+        bc = 1   # grow needs ny >= n + 15
+    elif case == 'narr':
+        narr = -1
+    x = np.zeros(512)
+    y = np.full(512, 3.0)
+    k = libconv.MagicFilter1d()
+    with pytest.raises(libconv.LibconvError):
+        k.run(d, idim, n, bc, nx, ny, narr, x, y, 1.0, 0.0, 0.0)
+    assert (y == 3.0).all()  # untouched on error
+
+
+def test_stream_api_rejects_bad_precision(lib):
+    import libconv
+    with pytest.raises(libconv.LibconvError):
+        libconv._rc(libconv.lib().lc_conv_1d(3, 0, 1, 0, (ctypes.c_int * 1)(8), 0, (ctypes.c_int * 1)(8),
+                                             (ctypes.c_int * 1)(8), 1, None, None, 1.0, 0.0, 0.0, None, None))

@@ -1,0 +1,250 @@
+"""GPU parity: the HIP library through its C ABI against the FP64 oracle.
+
+Bar: FP64 within 1e-13 relative max-norm of the oracle (the north-star
+tolerance; the reference's own tests use 1e-14 for MagicFilter, 1e-12 for
+Kinetic, 1e-11 for wavelets); FP32 storage and arithmetic within 2e-6
+(MagicFilter / wavelets) and 1e-5 (Kinetic, whose sum of |taps| is 9.9) of
+the FP64 oracle, relative to the oracle's max-norm; FP64 energies within
+1e-12 relative.  Inputs are seeded U[-1, 1].
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope='module')
+def lc(gpu):
+    import libconv
+    libconv.lib()
+    return libconv
+
+
+def conv_sizes(fil, bc, n, idim):
+    nin, nout = O.conv_extents(fil, bc, n[idim])
+    nx, ny = list(n), list(n)
+    nx[idim], ny[idim] = nin, nout
+    return nx, ny
+
+
+FILS = {'md': O.MD, 'imd': O.IMD, 'd2': O.D2, 'd1': O.D1}
+
+
+# ------------------------------------------------------------------ config 1
+def test_config1_magicfilter_dim0_32cube(lc):
+    """MagicFilter sweep along dim 0 of a 32^3 FP64 grid, periodic BC."""
+    rng = np.random.default_rng(1)
+    n = [32, 32, 32]
+    x = rng.uniform(-1, 1, 32 ** 3)
+    y = np.zeros_like(x)
+    lc.MagicFilter1d().run(3, 0, n, lc.BC.PERIODIC, n, n, 1, x, y, 1.0, 0.0, 0.0)
+    ref = np.zeros_like(x)
+    O.s0s0_1d(O.MD, 0, n, 0, n, n, 1, x, ref)
+    assert rel(y, ref) < TOL64
+
+
+# ------------------------------------------------------------------ 1D sweeps
+@pytest.mark.parametrize('fil', ['md', 'imd', 'd2', 'd1'])
+@pytest.mark.parametrize('bc', [0, 1, -1])
+@pytest.mark.parametrize('idim', [0, 1, 2])
+def test_conv_1d_sweeps(lc, fil, bc, idim):
+    rng = np.random.default_rng(100 + 7 * idim + bc)
+    n = [19, 23, 17]
+    nx, ny = conv_sizes(FILS[fil], bc, n, idim)
+    nx[idim] += 1  # padded leading dims
+    ny[idim] += 2
+    narr = 2
+    nin = int(np.prod(nx)) * narr
+    nout = int(np.prod(ny)) * narr
+    x = rng.uniform(-1, 1, nin)
+    y0 = rng.uniform(-1, 1, nout)
+    for a, a_x, a_y in ((1.0, 0.0, 0.0), (-2.5, 0.75, 0.5)):
+        y = y0.copy()
+        ref = y0.copy()
+        if fil == 'd2':
+            st = lc.Kinetic1d().run(3, idim, n, bc, nx, ny, narr, x, y, a, a_x, a_y, 0.0)
+            dot = st['reference_return']['dot_in']
+            rdot = O.s0s0_1d(O.D2, idim, n, bc, nx, ny, narr, x, ref, a, a_x, a_y, want_dot=True)
+            assert abs(dot - rdot) <= 1e-12 * abs(rdot)
+        else:
+            k = {'md': lc.MagicFilter1d, 'imd': lc.InverseMagicFilter1d}.get(fil)
+            if k is None:
+                k = lambda: lc.LibconvKernel('s0s0_1d_sym8_d1')
+            k().run(3, idim, n, bc, nx, ny, narr, x, y, a, a_x, a_y)
+            O.s0s0_1d(FILS[fil], idim, n, bc, nx, ny, narr, x, ref, a, a_x, a_y)
+        assert rel(y, ref) < TOL64, (fil, bc, idim, a)
+
+
+def test_a_y_zero_does_not_read_y(lc):
+    n = [16, 9, 5]
+    x = np.random.default_rng(2).uniform(-1, 1, int(np.prod(n)))
+    y = np.full_like(x, np.nan)
+    lc.MagicFilter1d().run(3, 1, n, 0, n, n, 1, x, y, 1.0, 0.0, 0.0)
+    assert np.isfinite(y).all()
+
+
+def test_small_n_below_filter_width(lc):
+    """n < filter width: true periodic wrap and both-sided grow clamping."""
+    rng = np.random.default_rng(3)
+    for nn in (1, 3, 9, 14):
+        for bc in (0, 1):
+            n = [nn, 6]
+            nx, ny = conv_sizes(O.D2, bc, n, 0)
+            x = rng.uniform(-1, 1, int(np.prod(nx)))
+            y = np.zeros(int(np.prod(ny)))
+            ref = y.copy()
+            lc.LibconvKernel('s0s0_1d_sym8_d2').run(2, 0, n, bc, nx, ny, 1, x, y, 1.0, 0.0, 0.0)
+            O.s0s0_1d(O.D2, 0, n, bc, nx, ny, 1, x, ref)
+            assert rel(y, ref) < TOL64, (nn, bc)
+
+
+@pytest.mark.parametrize('comb', [False, True])
+@pytest.mark.parametrize('direction', ['decompose', 'recompose'])
+@pytest.mark.parametrize('bc', [0, 1, -1])
+@pytest.mark.parametrize('idim', [0, 1, 2])
+def test_wavelet_1d(lc, comb, direction, bc, idim):
+    rng = np.random.default_rng(200 + idim + 5 * bc)
+    wav = O.SYMICOMB8 if comb else O.SYM8
+    n = [10, 13, 9]
+    nin, nout = O.wavelet_extents(wav, bc, n[idim])
+    nx, ny = list(n), list(n)
+    nx[idim], ny[idim] = nin + 1, nout
+    narr = 2
+    x = rng.uniform(-1, 1, int(np.prod([2 * v for v in nx])) * narr)
+    y0 = rng.uniform(-1, 1, int(np.prod([2 * v for v in ny])) * narr)
+    for a, a_y in ((1.0, 0.0), (1.3, -0.4)):
+        y, ref = y0.copy(), y0.copy()
+        lc.Wavelet1d(direction, combined=comb).run(3, idim, n, bc, nx, ny, narr, x, y, a, a_y)
+        f = O.s0s1_1d if direction == 'decompose' else O.s1s0_1d
+        f(wav, idim, n, bc, nx, ny, narr, x, ref, a, a_y)
+        assert rel(y, ref) < TOL64
+
+
+# ------------------------------------------------------------------ 3D operators
+@pytest.mark.parametrize('bcs', [(0, 0, 0), (1, 1, 1), (-1, -1, -1), (0, 1, -1)])
+@pytest.mark.parametrize('inverse', [False, True])
+def test_magicfilter_3d(lc, bcs, inverse):
+    rng = np.random.default_rng(300)
+    fil = O.IMD if inverse else O.MD
+    n = [21, 18, 23]
+    nx, ny = [], []
+    for i in range(3):
+        a, b = O.conv_extents(fil, bcs[i], n[i])
+        nx.append(a)
+        ny.append(b + (2 if bcs[i] == 0 else 0))  # padded output as MagicFilter_test2.rb
+    narr = 2
+    x = rng.uniform(-1, 1, int(np.prod(nx)) * narr)
+    y = np.zeros(int(np.prod(ny)) * narr)
+    ref = y.copy()
+    lc.MagicFilter(inverse=inverse).run(3, n, list(bcs), nx, ny, narr, x, y, None, None)
+    O.s0s0_md(fil, n, list(bcs), nx, ny, narr, x, ref)
+    assert rel(y, ref) < TOL64
+
+
+@pytest.mark.parametrize('bcs', [(0, 0, 0), (1, 1, 1), (-1, -1, -1), (1, 0, -1)])
+def test_kinetic_3d_with_energies(lc, bcs):
+    rng = np.random.default_rng(400)
+    n = [17, 20, 15]
+    nx = [v + 28 if b == -1 else v for v, b in zip(n, bcs)]
+    ny = [v + 28 if b == 1 else v for v, b in zip(n, bcs)]
+    x = rng.uniform(-1, 1, int(np.prod(nx)))
+    y0 = rng.uniform(-1, 1, int(np.prod(ny)))
+    h = (0.3, 0.35, 0.4)
+    a = [-0.5 / v ** 2 for v in h]
+    y, ref = y0.copy(), y0.copy()
+    st = lc.Kinetic(dot=True).run(3, n, list(bcs), x, y, None, None, a, 0.5, 0.25)
+    rdot = O.s0s0_1ds(O.D2, n, list(bcs), nx, ny, 1, a, 0.5, 0.25, x, ref)
+    assert rel(y, ref) < TOL64
+    assert np.abs(st['reference_return']['dot_in'] - rdot).max() <= 1e-12 * np.abs(rdot).max()
+
+
+@pytest.mark.parametrize('bc', [0, 1, -1])
+def test_wavelet_3d_and_round_trip(lc, bc):
+    rng = np.random.default_rng(500)
+    n = [9, 11, 8]
+    nin = [v + 7 if bc == -1 else v for v in n]
+    nout = [v + 7 if bc == 1 else v for v in n]
+    s = rng.uniform(-1, 1, int(np.prod([2 * v for v in nin])))
+    c = np.zeros(int(np.prod([2 * v for v in nout])))
+    ref = c.copy()
+    lc.Wavelet('decompose').run(3, n, [bc] * 3, s, c, None, None)
+    O.s0s1_md(O.SYM8, n, [bc] * 3, nin, nout, 1, s, ref)
+    assert rel(c, ref) < TOL64
+    if bc == 0:
+        back = np.zeros_like(s)
+        lc.Wavelet('recompose').run(3, n, [0, 0, 0], c, back, None, None)
+        assert rel(back, s) < TOL64
+
+
+def test_wavelet_round_trip_config3_shape(lc, gpu):
+    """sym8 DWT then IDWT on n = (96, 96, 96) pairs, periodic: identity."""
+    torch = gpu
+    n = [96, 96, 96]
+    g = torch.Generator(device='cuda').manual_seed(3)
+    s = torch.rand(192 ** 3, dtype=torch.float64, device='cuda', generator=g) * 2 - 1
+    c = torch.empty_like(s)
+    back = torch.empty_like(s)
+    lc.wavelet_md(0, False, n, [0, 0, 0], n, n, 1, s, c)
+    lc.wavelet_md(0, True, n, [0, 0, 0], n, n, 1, c, back)
+    torch.cuda.synchronize()
+    err = ((back - s).abs().max() / s.abs().max()).item()
+    assert err < TOL64
+    # orthonormal: the analysis keeps the 2-norm
+    assert abs(c.norm().item() / s.norm().item() - 1) < 1e-13
+
+
+# ------------------------------------------------------------------ FP32 form
+def test_fp32_forms_against_fp64_oracle(lc):
+    rng = np.random.default_rng(600)
+    n = [24, 20, 22]
+    x = rng.uniform(-1, 1, int(np.prod(n)))
+    y = np.zeros(int(np.prod(n)), np.float32)
+    ref = np.zeros(int(np.prod(n)))
+    lc.MagicFilter(fp32=True).run(3, n, [0, 0, 0], n, n, 1, x.astype(np.float32), y, None, None)
+    O.s0s0_md(O.MD, n, [0, 0, 0], n, n, 1, x, ref)
+    assert rel(y, ref) < 2e-6
+    a = [-5.5, -4.0, -3.0]
+    yk = np.zeros(int(np.prod(n)), np.float32)
+    st = lc.Kinetic(fp32=True, dot=True).run(3, n, [0, 0, 0], x.astype(np.float32), yk, None, None,
+                                            a, 0.0, 0.0)
+    refk = np.zeros(int(np.prod(n)))
+    rdot = O.s0s0_1ds(O.D2, n, [0, 0, 0], n, n, 1, a, 0.0, 0.0, x, refk)
+    assert rel(yk, refk) < 1e-5
+    # FP64 energies over FP32 data: the error is the data's rounding only
+    assert np.abs(st['reference_return']['dot_in'] - rdot).max() <= 1e-6 * np.abs(rdot).max()
+
+
+# ------------------------------------------------------------------ stream API
+def test_stream_api_device_tensors(lc, gpu):
+    torch = gpu
+    rng = np.random.default_rng(700)
+    n = [40, 36, 33]
+    bc = [1, 1, 1]
+    ny1 = [v + 15 for v in n]
+    ny2 = [v + 28 for v in ny1]
+    x = rng.uniform(-1, 1, int(np.prod(n)))
+    xd = torch.from_numpy(x).cuda()
+    y1 = torch.zeros(int(np.prod(ny1)), dtype=torch.float64, device='cuda')
+    y2 = torch.zeros(int(np.prod(ny2)), dtype=torch.float64, device='cuda')
+    dot = torch.zeros(3, dtype=torch.float64, device='cuda')
+    a = [-0.5 / 0.3 ** 2] * 3
+    s = torch.cuda.current_stream()
+    lc.conv_md(0, n, bc, n, ny1, 1, xd, y1, stream=s)
+    lc.conv_1ds(2, ny1, bc, ny1, ny2, 1, a, 0.0, 0.0, y1, y2, dot=dot, stream=s)
+    torch.cuda.synchronize()
+    r1 = np.zeros(int(np.prod(ny1)))
+    O.s0s0_md(O.MD, n, bc, n, ny1, 1, x, r1)
+    r2 = np.zeros(int(np.prod(ny2)))
+    rdot = O.s0s0_1ds(O.D2, ny1, bc, ny1, ny2, 1, a, 0.0, 0.0, r1, r2)
+    assert rel(y1.cpu().numpy(), r1) < TOL64
+    assert rel(y2.cpu().numpy(), r2) < TOL64
+    assert np.abs(dot.cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
