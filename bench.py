@@ -137,23 +137,20 @@ def oracle_chain(O, w, x):
     return y1, y2, dot
 
 
-def cpu_baseline(O, np):
-    """The oracle's OpenMP chain on the 128^3-output shrink sample (x 171^3):
-    one warm-up, then the min of 3 runs (BigDFT/MagicFilter_test.rb:69-79
-    keeps the min of repeated runs)."""
-    n2 = 128
-    w = dict(bc=-1, nx=n2 + 43, n1=n2 + 28, ny1=n2 + 28, n2=n2, ny2=n2, pts=n2 ** 3)
-    rng = np.random.default_rng(5)
-    x = rng.uniform(-1, 1, w['nx'] ** 3)
+def cpu_baseline(O, np, w, x):
+    """The oracle's OpenMP chain on the same workload and input: one warm-up,
+    then the min of 5 runs (BigDFT/MagicFilter_test.rb:69-79 keeps the min
+    of repeated runs)."""
     oracle_chain(O, w, x)
     best = 1e30
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         oracle_chain(O, w, x)
         best = min(best, time.perf_counter() - t0)
     return dict(value=w['pts'] / best / 1e9, unit='Gpts/s', cores=O.num_threads(), kind='port',
-                sample='MF3D->K3D chain at 128^3 output, free shrink (x 171^3), FP64, min of 3 after '
-                       '1 warm-up; oracle/liblc_oracle.so with OpenMP', seconds=best)
+                sample='the full workload (%s), FP64, min of 5 after 1 warm-up; '
+                       'oracle/liblc_oracle.so (C restatement, OpenMP)' % w['desc'],
+                seconds=round(best, 4))
 
 
 def main():
@@ -232,7 +229,7 @@ def main():
                                      'ekin_rel': float(abs(r32['ekin'].sum() - ds.sum()) / abs(ds.sum()))}
     if not args.no_cpu:
         import oracle as O
-        out['cpu_baseline'] = cpu_baseline(O, np)
+        out['cpu_baseline'] = cpu_baseline(O, np, w, r64['x'])
     print(json.dumps(out))
     return 0
 

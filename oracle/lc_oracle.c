@@ -347,55 +347,53 @@ static int compose_md(int kind, int id, int d, const int *n, const int *bc,
     if (d < 1 || d > 16 || narr < 0) return -1;
     int mul = kind == 0 ? 1 : 2;
     long ntot_in = prod(nx, d, mul), ntot_out = prod(ny, d, mul);
+    if (a_x != 0.0 && ntot_in != ntot_out) return -1;
     /* largest intermediate */
-    long wmax = 0;
+    long wmax = 1;
     int cur[16], nxt[16];
     for (int i = 0; i < d; ++i) cur[i] = nx[i];
-    for (int s = 0; s < d; ++s) {
+    for (int s = 0; s < d - 1; ++s) {
         for (int i = 0; i < d; ++i) nxt[i] = cur[i];
         nxt[s] = ny[s];
         long p = prod(nxt, d, mul);
         if (p > wmax) wmax = p;
         for (int i = 0; i < d; ++i) cur[i] = nxt[i];
     }
-    double *w1 = (double *)malloc(sizeof(double) * (size_t)wmax);
-    double *w2 = (double *)malloc(sizeof(double) * (size_t)wmax);
-    double *res = (double *)malloc(sizeof(double) * (size_t)ntot_out);
-    if (!w1 || !w2 || !res) { free(w1); free(w2); free(res); return -1; }
+    /* intermediates start zeroed: their padding (storage beyond the logical
+     * output) is never mixed into valid outputs, but stays finite */
+    double *w1 = (double *)calloc((size_t)wmax, sizeof(double));
+    double *w2 = (double *)calloc((size_t)wmax, sizeof(double));
+    if (!w1 || !w2) { free(w1); free(w2); return -1; }
     int rc = 0;
     for (int j = 0; j < narr && rc == 0; ++j) {
         const double *src = x + j * ntot_in;
+        double *ya = y + j * ntot_out;
         for (int i = 0; i < d; ++i) cur[i] = nx[i];
         for (int s = 0; s < d && rc == 0; ++s) {
-            double *dst = (s == d - 1) ? res : ((s % 2 == 0) ? w1 : w2);
+            const int last = s == d - 1;
+            double *dst = last ? ya : ((s % 2 == 0) ? w1 : w2);
+            double sa = last ? a : 1.0, say = last ? a_y : 0.0;
             int nxa[16], nya[16];
             for (int i = 0; i < d; ++i) { nxa[i] = cur[i]; nya[i] = cur[i]; }
             nya[s] = ny[s];
             if (kind == 0)
-                rc = lco_s0s0_1d(id, d, s, n, bc[s], nxa, nya, 1, src, dst, 1.0, 0.0, 0.0, NULL);
+                rc = lco_s0s0_1d(id, d, s, n, bc[s], nxa, nya, 1, src, dst, sa, 0.0, say, NULL);
             else if (kind == 1)
-                rc = lco_s0s1_1d(id, d, s, n, bc[s], nxa, nya, 1, src, dst, 1.0, 0.0);
+                rc = lco_s0s1_1d(id, d, s, n, bc[s], nxa, nya, 1, src, dst, sa, say);
             else
-                rc = lco_s1s0_1d(id, d, s, n, bc[s], nxa, nya, 1, src, dst, 1.0, 0.0);
+                rc = lco_s1s0_1d(id, d, s, n, bc[s], nxa, nya, 1, src, dst, sa, say);
             cur[s] = ny[s];
             src = dst;
         }
         if (rc) break;
-        /* epilogue: y = a*res + a_x*x + a_y*y (a_x only when shapes agree) */
-        double *ya = y + j * ntot_out;
-        const double *xa = x + j * ntot_in;
-        int use_ax = (a_x != 0.0) && ntot_in == ntot_out;
-        if (a_x != 0.0 && !use_ax) { rc = -1; break; }
+        /* y += a_x x (same shape only) */
+        if (a_x != 0.0) {
+            const double *xa = x + j * ntot_in;
 #pragma omp parallel for schedule(static)
-        for (long p = 0; p < ntot_out; ++p) {
-            double t = res[p];
-            if (a != 1.0) t = t * a;
-            if (use_ax) t = xa[p] * a_x + t;
-            if (a_y != 0.0) t = a_y * ya[p] + t;
-            ya[p] = t;
+            for (long p = 0; p < ntot_out; ++p) ya[p] = xa[p] * a_x + ya[p];
         }
     }
-    free(w1); free(w2); free(res);
+    free(w1); free(w2);
     return rc;
 }
 

@@ -104,7 +104,9 @@ def test_small_n_below_filter_width(lc):
             ref = y.copy()
             lc.LibconvKernel('s0s0_1d_sym8_d2').run(2, 0, n, bc, nx, ny, 1, x, y, 1.0, 0.0, 0.0)
             O.s0s0_1d(O.D2, 0, n, bc, nx, ny, 1, x, ref)
-            assert rel(y, ref) < TOL64, (nn, bc)
+            # n = 1 periodic maps x to x * sum(D2) ~ 1e-16: compare on the input scale
+            scale = max(np.abs(ref).max(), np.abs(x).max())
+            assert np.abs(y - ref).max() < TOL64 * scale, (nn, bc)
 
 
 @pytest.mark.parametrize('comb', [False, True])
@@ -147,7 +149,11 @@ def test_magicfilter_3d(lc, bcs, inverse):
     ref = y.copy()
     lc.MagicFilter(inverse=inverse).run(3, n, list(bcs), nx, ny, narr, x, y, None, None)
     O.s0s0_md(fil, n, list(bcs), nx, ny, narr, x, ref)
-    assert rel(y, ref) < TOL64
+    # storage beyond the logical output box (padding) is unspecified
+    nout = [O.conv_extents(fil, b, v)[1] for b, v in zip(bcs, n)]
+    shape = (narr, ny[2], ny[1], ny[0])
+    box = (slice(None), slice(0, nout[2]), slice(0, nout[1]), slice(0, nout[0]))
+    assert rel(y.reshape(shape)[box], ref.reshape(shape)[box]) < TOL64
 
 
 @pytest.mark.parametrize('bcs', [(0, 0, 0), (1, 1, 1), (-1, -1, -1), (1, 0, -1)])
