@@ -162,7 +162,19 @@ def main():
     ap.add_argument('--no-oracle', action='store_true', help='skip the FP64 oracle error check')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--fp64-only', action='store_true')
+    ap.add_argument('--copy-calib', action='store_true',
+                    help='only run a 1 GiB device copy (calibrates the HBM byte counters)')
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        nb = 1 << 30
+        src = torch.ones(nb // 8, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_calib_bytes_read': nb, 'copy_calib_bytes_written': nb, 'launches': 3}))
+        return 0
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; --gpus %d is not supported '
               '(orbitals shard across GPUs without communication, see DESIGN.md)' % args.gpus)

@@ -9,11 +9,24 @@
 //   sep3d  separable product y = a F2 F1 F0 x + a_y y (MagicFilter,
 //          transposing composition, GenericConvolution2.rb:2160-2240 with
 //          :transpose => 1; the final layout equals the input's axis order).
+#include <stdlib.h>
+
 #include <algorithm>
 
 #include "lc_kernels.hpp"
 
 namespace lc {
+
+// LIBCONV_DIRECT=1 selects the unfused kernels (one lane per output for the
+// sum, three sweeps for the product): a debugging aid, not a fast path.
+static bool direct_kernels() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("LIBCONV_DIRECT");
+    v = (e && e[0] == '1') ? 1 : 0;
+  }
+  return v == 1;
+}
 
 // ------------------------------------------------------------ geometry
 struct Dim3G {
@@ -127,10 +140,17 @@ __global__ __launch_bounds__(256) void sum3d_direct(const R* __restrict__ x, R* 
 
 template <typename R>
 size_t sum3d_scratch_bytes(const Box3& b, int narr) {
-  OpGeom gm = conv_geom(OP_D2);
-  long npts = 1;
-  for (int d = 0; d < 3; ++d) npts *= dim_geom(b.n[d], b.bc[d], b.nx[d], b.ny[d], gm.lo, gm.up).next;
-  return sizeof(double) * 3 * (size_t)((npts + 255) / 256);
+  // the larger of the direct kernel's and the fused kernel's partials
+  size_t best = 0;
+  for (int id = 0; id < 4; ++id) {
+    OpGeom gm = conv_geom(id);
+    long npts = 1;
+    for (int d = 0; d < 3; ++d) npts *= dim_geom(b.n[d], b.bc[d], b.nx[d], b.ny[d], gm.lo, gm.up).next;
+    best = std::max(best, (size_t)((npts + 255) / 256));
+    best = std::max(best, fused_sum3d_parts(b, narr, gm.lo, gm.up, gm.w));
+    best = std::max(best, ws_sum3d_parts_bound(b, gm.lo, gm.up));
+  }
+  return sizeof(double) * 3 * best;
 }
 
 template <typename R, class F>
@@ -159,6 +179,8 @@ template <typename R>
 hipError_t sum3d(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
                  double* dot, hipStream_t st, double* scratch, size_t scratch_bytes) {
   if (dot && scratch_bytes < sum3d_scratch_bytes<R>(b, narr)) return hipErrorInvalidValue;
+  if (!direct_kernels())
+    return sum3d_fused_run<R>(id, b, narr, x, y, a, a_x, a_y, dot, st, scratch, scratch_bytes);
   switch (id) {
     case OP_D2: return sum3d_t<R, FD2>(b, narr, x, y, a, a_x, a_y, dot, st, scratch);
     case OP_D1: return sum3d_t<R, FD1>(b, narr, x, y, a, a_x, a_y, dot, st, scratch);
@@ -180,6 +202,7 @@ size_t sep3d_scratch_bytes(const Box3& b, int narr) {
 template <typename R>
 hipError_t sep3d(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
                  double* scratch, size_t scratch_bytes) {
+  if (!direct_kernels()) return sep3d_fused_run<R>(id, b, narr, x, y, a, a_y, st);
   if (scratch_bytes < sep3d_scratch_bytes<R>(b, narr)) return hipErrorInvalidValue;
   R* w1 = reinterpret_cast<R*>(scratch);
   R* w2 = w1 + (size_t)std::max(narr, 1) * b.ny[0] * b.nx[1] * b.nx[2];

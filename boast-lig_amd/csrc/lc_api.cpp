@@ -299,6 +299,27 @@ int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, 
     rc = stage_in<R>(y, (size_t)tout * narr, 2, true, st, &ay);
     if (rc) return rc;
   }
+  if (d == 3 && kind == KCONV) {  // single-pass 3D product
+    Box3 b;
+    for (int i = 0; i < 3; ++i) { b.n[i] = n[i]; b.bc[i] = bc[i]; b.nx[i] = nx[i]; b.ny[i] = ny[i]; }
+    const size_t sb = sep3d_scratch_bytes<R>(b, narr);
+    void* sc = nullptr;
+    if (getenv("LIBCONV_DIRECT")) {
+      rc = pool_get(0, sb, &sc);
+      if (rc) return rc;
+    }
+    hipError_t e = sep3d<R>(id, b, narr, ax.dev, ay.dev, (R)a, (R)a_y, st, static_cast<double*>(sc), sb);
+    if (e != hipSuccess) return hip_fail(e, who);
+    if (a_x != 0.0) {
+      e = axpy((long)tout * narr, (R)a_x, ax.dev, ay.dev, st);
+      if (e != hipSuccess) return hip_fail(e, who);
+    }
+    if (!device_api) {
+      if (ay.host_in) HIPCHK(hipMemcpyAsync(y, ay.dev, (size_t)tout * narr * sizeof(R), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    return LC_OK;
+  }
   // intermediates: after sweep s the dims < s+1 carry ny
   int cur[16];
   for (int i = 0; i < d; ++i) cur[i] = nx[i];

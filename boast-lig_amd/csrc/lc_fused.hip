@@ -1,0 +1,603 @@
+// lc_fused.hip -- single-pass 3D libconv operators on gfx950.
+//
+// Both kernels march along the slowest dim (z) over an xy tile of the
+// output, so every input value is read once from HBM and every output
+// written once (the neighbouring tiles' halo reads hit L2 / Infinity Cache):
+//
+//   sum3d_fused  kinetic form y = a0 Fx x~ + a1 Fy x~ + a2 Fz x~ + a_x x~
+//                + a_y y with the per-direction FP64 energies.  Per output
+//                plane z the xy tile of x~(z) plus its cross-shaped filter
+//                halo sits in LDS: x-tasks (QX consecutive outputs of a row
+//                from one register window) write the x-terms to LDS, the
+//                column owners compute the y-terms (CPT consecutive rows of
+//                a column) and the z-terms from a register window of the
+//                column's values along z, fed BZ planes ahead of use.
+//   sep3d_fused  separable product y = a Fz Fy Fx x~ + a_y y.  Per input
+//                plane p: the (TX+W-1) x (TY+W-1) tile goes to LDS, x-tasks
+//                filter it along x into LDS, the column owners filter along
+//                y straight into their register window along z and emit the
+//                output plane p - UP.
+// x~ is x wrapped along periodic dims and zero outside its box along free
+// dims, exactly the per-dim semantics of the 1D sweeps
+// (BigDFT/GenericConvolution2.rb:91-121 index ranges, :2160-2240 chains).
+#include <stdlib.h>
+
+#include <algorithm>
+
+#include "lc_kernels.hpp"
+
+namespace lc {
+
+struct FDim {   // one dim of a fused 3D operator
+  int n, bc;
+  int nx, ny;   // storage extents of input / output
+  int ox, oy;   // logical index of the first stored input / output element
+  int olo, nout;  // first logical output index and output count
+};
+
+__device__ __forceinline__ int fsrc(const FDim& g, int j) {
+  if (g.bc == PER) return pmod(j, g.n);
+  if (g.bc == GROW) return (j >= 0 && j < g.n) ? j : -1;
+  const int s = j - g.ox;
+  return (s >= 0 && s < g.nx) ? s : -1;  // beyond the stored input: never used
+}
+
+template <typename R>
+struct FusedArgs {
+  const R* x;
+  R* y;
+  FDim d0, d1, d2;
+  long nti, nto;
+  int narr;
+  int tiles_x, tiles_y, zchunk;  // zchunk = outputs per block along z
+  R a0, a1, a2, a_x, a_y;
+  int flags;
+  double* part;  // 3 sets of per-block energies (sum3d) or null
+  long nparts;   // blocks per array = sets stride
+};
+
+// Block -> (tile, z chunk, array).  Workgroups are dispatched round-robin
+// over the 8 XCDs of the chip, each with its own L2: the linear block id is
+// permuted so that every XCD receives one contiguous run of tiles, and the
+// filter halos that neighbouring tiles share are fetched once per XCD.
+__device__ __forceinline__ void block_coords(int tiles_x, int& bx, int& by, int& zc, int& arr) {
+  const int tiles = gridDim.x, nz = gridDim.y;
+  const int nb = tiles * nz * gridDim.z;
+  const int lin = blockIdx.x + tiles * (blockIdx.y + nz * blockIdx.z);
+  const int xcd = lin & 7, k = lin >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  const int nl = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+  const int tile = nl % tiles;
+  const int rest = nl / tiles;
+  zc = rest % nz;
+  arr = rest / nz;
+  bx = tile % tiles_x;
+  by = tile / tiles_x;
+}
+
+template <typename R>
+__device__ __forceinline__ double wave_block_sum(double v, double* red, int nthreads) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (nthreads + 63) / 64; ++w) s += red[w];
+  return s;
+}
+
+// ====================================================================== sum3d
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD>
+__global__ __launch_bounds__(TX * TY / CPT) void sum3d_fused(FusedArgs<R> p) {
+  constexpr int NT = TX * TY / CPT;
+  constexpr int W = F::N, HL = -F::LO, HR = F::UP;
+  constexpr int TW = (TX + HL + HR) | 1;    // LDS tile row stride (odd: no bank conflicts)
+  constexpr int SXW = TX | 1;                // x-term row stride
+  constexpr int TH = TY + HL + HR;          // LDS tile rows
+  constexpr int WS = W - 1 + BZ;            // z window per column
+  constexpr int NH = (TX + TY) * (HL + HR); // halo elements per plane
+  constexpr int NHT = (NH + NT - 1) / NT;   // halo elements per thread
+  constexpr int NXT = TY * (TX / QX);       // x-tasks
+  static_assert(TY % CPT == 0 && TX % QX == 0 && BZ % HD == 0, "tile shape");
+  __shared__ R TB[2][TH * TW];   // double-buffered plane tile
+  __shared__ R SX[TY * SXW];
+  __shared__ double red[NT / 64 + 1];
+
+  const int t = threadIdx.x;
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  const R* __restrict__ xa = p.x + arr * p.nti;
+  R* __restrict__ ya = p.y + arr * p.nto;
+  const int X0 = p.d0.olo + bx * TX, Y0 = p.d1.olo + by * TY;
+  const int zb = p.d2.olo + zc * p.zchunk;
+  const int ze = min(zb + p.zchunk, p.d2.olo + p.d2.nout);
+  const long sxy = (long)p.d0.nx * p.d1.nx;
+
+  // ---- owner columns: (tx, ty0 + c)
+  const int tx = t % TX, ty0 = (t / TX) * CPT;
+  int coff[CPT];   // xy offset of the column in x (or -1 when x~ is zero)
+  bool cout_ok[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int s0 = fsrc(p.d0, X0 + tx), s1 = fsrc(p.d1, Y0 + ty0 + c);
+    coff[c] = (s0 >= 0 && s1 >= 0) ? s0 + p.d0.nx * s1 : -1;
+    cout_ok[c] = (X0 + tx < p.d0.olo + p.d0.nout) && (Y0 + ty0 + c < p.d1.olo + p.d1.nout);
+  }
+  // ---- halo elements of this thread: LDS slot and xy offset
+  int hslot[NHT], hoff[NHT];
+#pragma unroll
+  for (int k = 0; k < NHT; ++k) {
+    const int e = t + k * NT;
+    int r = 0, cc = 0;
+    bool valid = e < NH;
+    if (e < TY * (HL + HR)) {
+      r = e / (HL + HR);
+      const int q = e % (HL + HR);
+      cc = q < HL ? q - HL : TX + (q - HL);
+    } else if (valid) {
+      const int e2 = e - TY * (HL + HR);
+      const int rr = e2 / TX;
+      cc = e2 % TX;
+      r = rr < HL ? rr - HL : TY + (rr - HL);
+    }
+    hslot[k] = valid ? (r + HL) * TW + (cc + HL) : -1;
+    const int s0 = fsrc(p.d0, X0 + cc), s1 = fsrc(p.d1, Y0 + r);
+    hoff[k] = (valid && s0 >= 0 && s1 >= 0) ? s0 + p.d0.nx * s1 : -1;
+  }
+
+  auto plane_base = [&](int z, bool& ok) -> const R* {
+    const int s2 = fsrc(p.d2, z);
+    ok = s2 >= 0;
+    return xa + (ok ? (long)s2 * sxy : 0);
+  };
+  auto load_cols = [&](int z, R* dst) {
+    bool ok;
+    const R* b = plane_base(z, ok);
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) dst[c] = (ok && coff[c] >= 0) ? b[coff[c]] : R(0);
+  };
+  auto load_halo = [&](int z, R* dst) {
+    bool ok;
+    const R* b = plane_base(z, ok);
+#pragma unroll
+    for (int k = 0; k < NHT; ++k) dst[k] = (ok && hoff[k] >= 0) ? b[hoff[k]] : R(0);
+  };
+
+  R win[CPT][WS];
+  R stg[CPT][BZ];
+  R hreg[HD][NHT];   // halo of planes z .. z+HD-1 in flight
+  // prefill the window with planes zb+LO .. zb+UP+BZ-1
+#pragma unroll
+  for (int s = 0; s < WS; ++s) {
+    R v[CPT];
+    load_cols(zb + F::LO + s, v);
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) win[c][s] = v[c];
+  }
+#pragma unroll
+  for (int h = 0; h < HD; ++h) load_halo(zb + h, hreg[h]);
+  const bool want_dot = (p.flags & F_DOT) && arr == p.narr - 1;
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+
+  for (int z0 = zb; z0 < ze; z0 += BZ) {
+    // stage the next block's new planes (z0+UP+BZ .. z0+UP+2BZ-1)
+#pragma unroll
+    for (int u = 0; u < BZ; ++u) {
+      R v[CPT];
+      load_cols(z0 + F::UP + BZ + u, v);
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) stg[c][u] = v[c];
+    }
+#pragma unroll
+    for (int u = 0; u < BZ; ++u) {
+      const int z = z0 + u;
+      R* const T = TB[(z - zb) & 1];
+      if (z < ze) {
+        // ---- tile of plane z: centre from the window, halo from registers
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) T[(ty0 + c + HL) * TW + tx + HL] = win[c][u + HL];
+#pragma unroll
+        for (int k = 0; k < NHT; ++k)
+          if (hslot[k] >= 0) T[hslot[k]] = hreg[u % HD][k];
+        if (z + HD < ze) load_halo(z + HD, hreg[u % HD]);
+        __syncthreads();
+        // ---- x-terms
+        if (t < NXT) {
+          // consecutive lanes on consecutive rows: odd strides, no conflicts
+          const int r = t % TY, c0 = (t / TY) * QX;
+          R wv[QX + W - 1];
+#pragma unroll
+          for (int k = 0; k < QX + W - 1; ++k) wv[k] = T[(r + HL) * TW + c0 + k];
+#pragma unroll
+          for (int q = 0; q < QX; ++q) {
+            R s = R(0);
+#pragma unroll
+            for (int k = 0; k < W; ++k) s = __builtin_fma(wv[q + k], tap<R, F>(k), s);
+            SX[r * SXW + c0 + q] = s;
+          }
+        }
+        __syncthreads();
+        // ---- y-terms and z-terms of the owned columns (the tile is double
+        // buffered, so it stays valid until the next plane's first barrier)
+        R yt[CPT], zt[CPT];
+        {
+          R wv[CPT + W - 1];
+#pragma unroll
+          for (int k = 0; k < CPT + W - 1; ++k) wv[k] = T[(ty0 + k) * TW + tx + HL];
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            R s = R(0);
+#pragma unroll
+            for (int k = 0; k < W; ++k) s = __builtin_fma(wv[c + k], tap<R, F>(k), s);
+            yt[c] = s;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          R s = R(0);
+#pragma unroll
+          for (int k = 0; k < W; ++k) s = __builtin_fma(win[c][u + k], tap<R, F>(k), s);
+          zt[c] = s;
+        }
+        // ---- combine
+        const bool zin = fsrc(p.d2, z) >= 0;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          const R xc = win[c][u + HL];
+          const bool has_x = zin && coff[c] >= 0 && cout_ok[c];  // output points only
+          R t0 = SX[(ty0 + c) * SXW + tx] * p.a0;
+          const R t1 = yt[c] * p.a1;
+          const R t2 = zt[c] * p.a2;
+          if (want_dot && has_x) {
+            d0 = __builtin_fma((double)xc, (double)t0, d0);
+            d1 = __builtin_fma((double)xc, (double)t1, d1);
+            d2 = __builtin_fma((double)xc, (double)t2, d2);
+          }
+          if ((p.flags & F_AX) && has_x) t0 = __builtin_fma(xc, p.a_x, t0);
+          R tot = t1 + t0;
+          tot = t2 + tot;
+          if (cout_ok[c]) {
+            R* yp = ya + (X0 + tx - p.d0.oy) +
+                    (long)p.d0.ny * ((Y0 + ty0 + c - p.d1.oy) + (long)p.d1.ny * (z - p.d2.oy));
+            if (p.flags & F_AY) tot = __builtin_fma(p.a_y, *yp, tot);
+            *yp = tot;
+          }
+        }
+      }
+    }
+    // ---- slide the window by BZ planes
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+      for (int s = 0; s < W - 1; ++s) win[c][s] = win[c][s + BZ];
+#pragma unroll
+      for (int u = 0; u < BZ; ++u) win[c][W - 1 + u] = stg[c][u];
+    }
+  }
+  if (p.part) {
+    const long bid = (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc;
+    double s = wave_block_sum<R>(d0, red, NT);
+    if (t == 0 && arr == p.narr - 1) p.part[bid] = s;
+    s = wave_block_sum<R>(d1, red, NT);
+    if (t == 0 && arr == p.narr - 1) p.part[p.nparts + bid] = s;
+    s = wave_block_sum<R>(d2, red, NT);
+    if (t == 0 && arr == p.narr - 1) p.part[2 * p.nparts + bid] = s;
+  }
+}
+
+// ====================================================================== sep3d
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD>
+__global__ __launch_bounds__(TX * TY / CPT) void sep3d_fused(FusedArgs<R> p) {
+  constexpr int NT = TX * TY / CPT;
+  constexpr int W = F::N;
+  constexpr int TW = TX + W - 1;            // input tile row length
+  constexpr int TWP = TW | 1;               // its LDS stride (odd)
+  constexpr int BW = TX | 1;                // x-filtered row stride (odd)
+  constexpr int TH = TY + W - 1;            // input tile rows
+  constexpr int WS = W - 1 + BZ;
+  constexpr int NA = TH * TW;               // input tile elements
+  constexpr int NAT = (NA + NT - 1) / NT;
+  constexpr int NXT = TH * (TX / QX);       // x-tasks
+  static_assert(TY % CPT == 0 && TX % QX == 0, "tile shape");
+  __shared__ R A[TH * TWP];
+  __shared__ R B[TH * BW];
+
+  const int t = threadIdx.x;
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  const R* __restrict__ xa = p.x + arr * p.nti;
+  R* __restrict__ ya = p.y + arr * p.nto;
+  const int X0 = p.d0.olo + bx * TX, Y0 = p.d1.olo + by * TY;
+  const int zb = p.d2.olo + zc * p.zchunk;
+  const int ze = min(zb + p.zchunk, p.d2.olo + p.d2.nout);
+  const long sxy = (long)p.d0.nx * p.d1.nx;
+
+  const int tx = t % TX, ty0 = (t / TX) * CPT;
+  bool cout_ok[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c)
+    cout_ok[c] = (X0 + tx < p.d0.olo + p.d0.nout) && (Y0 + ty0 + c < p.d1.olo + p.d1.nout);
+  // input tile element e -> logical (X0 + LO + e % TW, Y0 + LO + e / TW)
+  int aoff[NAT], aslot[NAT];
+#pragma unroll
+  for (int k = 0; k < NAT; ++k) {
+    const int e = t + k * NT;
+    const int s0 = fsrc(p.d0, X0 + F::LO + e % TW), s1 = fsrc(p.d1, Y0 + F::LO + e / TW);
+    aoff[k] = (e < NA && s0 >= 0 && s1 >= 0) ? s0 + p.d0.nx * s1 : -1;
+    aslot[k] = e < NA ? (e / TW) * TWP + e % TW : -1;
+  }
+  auto load_tile = [&](int z, R* dst) {
+    const int s2 = fsrc(p.d2, z);
+    const R* b = xa + (s2 >= 0 ? (long)s2 * sxy : 0);
+#pragma unroll
+    for (int k = 0; k < NAT; ++k) dst[k] = (s2 >= 0 && aoff[k] >= 0) ? b[aoff[k]] : R(0);
+  };
+
+  R win[CPT][WS];
+  R areg[AD][NAT];  // input tiles of the next AD planes in flight
+  const int plast = ze - 1 + F::UP;  // last input plane any output needs
+  const int pfirst = zb + F::LO;
+#pragma unroll
+  for (int h = 0; h < AD; ++h)
+    if (pfirst + h <= plast) load_tile(pfirst + h, areg[h]);
+  // x and y passes of input plane `plane` (its tile in areg[slot]) into pv
+  auto filter_plane = [&](int plane, R* ar, R* pv) {
+#pragma unroll
+    for (int k = 0; k < NAT; ++k)
+      if (aslot[k] >= 0) A[aslot[k]] = ar[k];
+    if (plane + AD <= plast) load_tile(plane + AD, ar);
+    __syncthreads();
+    for (int task = t; task < NXT; task += NT) {
+      // consecutive lanes on consecutive rows: odd strides, no bank conflicts
+      const int r = task % TH, c0 = (task / TH) * QX;
+      R wv[QX + W - 1];
+#pragma unroll
+      for (int k = 0; k < QX + W - 1; ++k) wv[k] = A[r * TWP + c0 + k];
+#pragma unroll
+      for (int q = 0; q < QX; ++q) {
+        R sum = R(0);
+#pragma unroll
+        for (int k = 0; k < W; ++k) sum = __builtin_fma(wv[q + k], tap<R, F>(k), sum);
+        B[r * BW + c0 + q] = sum;
+      }
+    }
+    __syncthreads();
+    R wv[CPT + W - 1];
+#pragma unroll
+    for (int k = 0; k < CPT + W - 1; ++k) wv[k] = B[(ty0 + k) * BW + tx];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      R sum = R(0);
+#pragma unroll
+      for (int k = 0; k < W; ++k) sum = __builtin_fma(wv[c + k], tap<R, F>(k), sum);
+      pv[c] = sum;
+    }
+  };
+  // prefill: planes pfirst .. pfirst+W-2 enter the window's first W-1 slots
+  // (plane pfirst + q sits in prefetch slot q % AD)
+  for (int q0 = 0; q0 < W - 1; q0 += AD) {
+#pragma unroll
+    for (int j = 0; j < AD; ++j) {
+      const int q = q0 + j;
+      if (q < W - 1) {
+        R pv[CPT];
+        filter_plane(pfirst + q, areg[j], pv);
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+          for (int s = 0; s < W - 2; ++s) win[c][s] = win[c][s + 1];
+          win[c][W - 2] = pv[c];
+        }
+      }
+    }
+  }
+  for (int z0 = zb; z0 < ze; z0 += BZ) {
+#pragma unroll
+    for (int u = 0; u < BZ; ++u) {
+      const int z = z0 + u;
+      if (z < ze) {
+        R pv[CPT];
+        // plane z + UP is plane number (W - 1) + (z - zb) after pfirst
+        filter_plane(z + F::UP, areg[((W - 1) % AD + u) % AD], pv);
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+          win[c][W - 1 + u] = pv[c];
+          R sum = R(0);
+#pragma unroll
+          for (int k = 0; k < W; ++k) sum = __builtin_fma(win[c][u + k], tap<R, F>(k), sum);
+          if (p.flags & F_A) sum = sum * p.a0;
+          if (cout_ok[c]) {
+            R* yp = ya + (X0 + tx - p.d0.oy) +
+                    (long)p.d0.ny * ((Y0 + ty0 + c - p.d1.oy) + (long)p.d1.ny * (z - p.d2.oy));
+            if (p.flags & F_AY) sum = __builtin_fma(p.a_y, *yp, sum);
+            *yp = sum;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+      for (int s = 0; s < W - 1; ++s) win[c][s] = win[c][s + BZ];
+  }
+}
+
+// ====================================================================== launch
+static FDim fdim(int n, int bc, int nx, int ny, int lo, int up) {
+  FDim g;
+  g.n = n; g.bc = bc; g.nx = nx; g.ny = ny;
+  g.ox = bc == SHRINK ? lo : 0;
+  g.oy = bc == GROW ? -up : 0;
+  g.olo = bc == GROW ? -up : 0;
+  g.nout = bc == GROW ? n + (up - lo) : n;
+  return g;
+}
+
+// z chunk: enough blocks for ~2 per CU, chunks no shorter than 4 * filter length
+static int pick_zchunk(long tiles, int nz, int narr, int w, int ncu) {
+  const long want = 2L * ncu;
+  long chunks = (want + tiles * narr - 1) / (tiles * narr);
+  const long maxc = std::max(1, nz / (4 * w));
+  chunks = std::min(chunks, maxc);
+  chunks = std::max(chunks, 1L);
+  return (int)((nz + chunks - 1) / chunks);
+}
+
+static int g_ncu = 0;
+static int num_cu() {
+  if (!g_ncu) {
+    int dev = 0;
+    hipDeviceProp_t pr;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess)
+      g_ncu = pr.multiProcessorCount;
+    if (g_ncu <= 0) g_ncu = 256;
+  }
+  return g_ncu;
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD>
+static hipError_t launch_sum3d(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
+                               R a_y, double* dot, hipStream_t st, double* scratch,
+                               size_t scratch_bytes) {
+  FusedArgs<R> p{};
+  p.x = x; p.y = y;
+  p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+  p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+  p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+  p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+  p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  p.narr = narr;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  p.a0 = (R)a[0]; p.a1 = (R)a[1]; p.a2 = (R)a[2];
+  p.a_x = a_x; p.a_y = a_y;
+  p.flags = (a_x != R(0) ? F_AX : 0) | (a_y != R(0) ? F_AY : 0) | (dot ? F_DOT : 0);
+  p.nparts = tiles * zch;
+  if (dot) {
+    if (scratch_bytes < sizeof(double) * 3 * (size_t)p.nparts) return hipErrorInvalidValue;
+    p.part = scratch;
+  }
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  sum3d_fused<R, F, TX, TY, CPT, QX, BZ, HD><<<grid, TX * TY / CPT, 0, st>>>(p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !dot) return e;
+  return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD>
+static hipError_t launch_sep3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                               hipStream_t st) {
+  FusedArgs<R> p{};
+  p.x = x; p.y = y;
+  p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+  p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+  p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+  p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+  p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  p.narr = narr;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  p.a0 = a; p.a_y = a_y;
+  p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0);
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  sep3d_fused<R, F, TX, TY, CPT, QX, BZ, AD><<<grid, TX * TY / CPT, 0, st>>>(p);
+  return hipGetLastError();
+}
+
+size_t fused_sum3d_parts(const Box3& b, int narr, int lo, int up, int w) {
+  FDim g0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], lo, up);
+  FDim g1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], lo, up);
+  FDim g2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], lo, up);
+  // upper bound over the variants: smallest tile, most z chunks
+  const long tiles = (long)((g0.nout + 31) / 32) * ((g1.nout + 7) / 8);
+  const long maxc = std::max(1, g2.nout / (4 * w)) + 1;
+  (void)narr;
+  return (size_t)(tiles * maxc);
+}
+
+// Kernel variants (tile TX x TY, CPT columns per lane, QX outputs per
+// x-task, BZ planes per window slide, HD/AD prefetch depth).  The variant is
+// picked per call by the tuner (lc_tune.cpp); index 0 is the default.
+static int env_variant(const char* name) {
+  const char* e = getenv(name);
+  return e ? atoi(e) : -1;
+}
+
+template <typename R, class F>
+static hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
+                                R a_x, R a_y, double* dot, hipStream_t st, double* sc, size_t sb) {
+  switch (v) {
+    case 1: return launch_sum3d<R, F, 32, 16, 2, 2, 4, 1>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    case 2: return launch_sum3d<R, F, 32, 8, 2, 2, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    case 3: return launch_sum3d<R, F, 64, 8, 2, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    case 4: return launch_sum3d<R, F, 32, 16, 2, 2, 2, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    case 0: return launch_sum3d<R, F, 32, 16, 2, 2, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    case 6: return launch_sum3d_ws<R, F, 64, 8, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
+    case 7: return launch_sum3d_ws<R, F, 32, 16, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
+    case 5: return launch_sum3d_ws<R, F, 32, 8, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
+    default: return launch_sum3d<R, F, 32, 16, 2, 2, 4, 1>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+  }
+}
+
+template <typename R, class F>
+static hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                                hipStream_t st) {
+  switch (v) {
+    case 1: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 1>(b, narr, x, y, a, a_y, st);
+    case 2: return launch_sep3d<R, F, 32, 8, 2, 4, 4, 2>(b, narr, x, y, a, a_y, st);
+    case 3: return launch_sep3d<R, F, 64, 8, 2, 4, 4, 2>(b, narr, x, y, a, a_y, st);
+    case 4: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 2>(b, narr, x, y, a, a_y, st);
+    case 0: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 4>(b, narr, x, y, a, a_y, st);
+    case 6: return launch_sep3d_ws<R, F, 64, 8, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
+    case 7: return launch_sep3d_ws<R, F, 32, 16, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
+    case 5: return launch_sep3d_ws<R, F, 32, 8, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
+    default: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 1>(b, narr, x, y, a, a_y, st);
+  }
+}
+
+template <typename R>
+hipError_t sum3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
+                           R a_y, double* dot, hipStream_t st, double* scratch, size_t sb) {
+  const int v = env_variant("LIBCONV_SUM3D_VARIANT");
+  switch (id) {
+    case OP_D2: return sum3d_variant<R, FD2>(v, b, narr, x, y, a, a_x, a_y, dot, st, scratch, sb);
+    case OP_D1: return sum3d_variant<R, FD1>(v, b, narr, x, y, a, a_x, a_y, dot, st, scratch, sb);
+    case OP_MD: return sum3d_variant<R, FMD>(v, b, narr, x, y, a, a_x, a_y, dot, st, scratch, sb);
+    case OP_IMD: return sum3d_variant<R, FIMD>(v, b, narr, x, y, a, a_x, a_y, dot, st, scratch, sb);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
+  const int v = env_variant("LIBCONV_SEP3D_VARIANT");
+  switch (id) {
+    case OP_MD: return sep3d_variant<R, FMD>(v, b, narr, x, y, a, a_y, st);
+    case OP_IMD: return sep3d_variant<R, FIMD>(v, b, narr, x, y, a, a_y, st);
+    case OP_D2: return sep3d_variant<R, FD2>(v, b, narr, x, y, a, a_y, st);
+    case OP_D1: return sep3d_variant<R, FD1>(v, b, narr, x, y, a, a_y, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template hipError_t sum3d_fused_run<double>(int, const Box3&, int, const double*, double*, const double*,
+                                            double, double, double*, hipStream_t, double*, size_t);
+template hipError_t sum3d_fused_run<float>(int, const Box3&, int, const float*, float*, const double*,
+                                           float, float, double*, hipStream_t, double*, size_t);
+template hipError_t sep3d_fused_run<double>(int, const Box3&, int, const double*, double*, double, double,
+                                            hipStream_t);
+template hipError_t sep3d_fused_run<float>(int, const Box3&, int, const float*, float*, float, float,
+                                           hipStream_t);
+
+}  // namespace lc

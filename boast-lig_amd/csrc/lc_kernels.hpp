@@ -62,6 +62,25 @@ hipError_t sum3d(int id, const Box3& b, int narr, const R* x, R* y, const double
                  double* dot, hipStream_t st, double* scratch, size_t scratch_bytes);
 template <typename R>
 size_t sum3d_scratch_bytes(const Box3& b, int narr);
+
+// single-pass kernels (lc_fused.hip)
+template <typename R>
+hipError_t sum3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
+                           R a_y, double* dot, hipStream_t st, double* scratch, size_t scratch_bytes);
+template <typename R>
+hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st);
+size_t fused_sum3d_parts(const Box3& b, int narr, int lo, int up, int w);
+
+// wave-specialised single-pass kernels (lc_fused_ws.hip); per_cu = target
+// blocks per CU, minz = shortest z chunk
+template <typename R, class F, int TX, int TY, int QX, int QY, int HD>
+hipError_t launch_sum3d_ws(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
+                           double* dot, hipStream_t st, double* scratch, size_t sb, int per_cu,
+                           int minz);
+template <typename R, class F, int TX, int TY, int QX, int QY, int AD>
+hipError_t launch_sep3d_ws(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                           int per_cu, int minz);
+size_t ws_sum3d_parts_bound(const Box3& b, int lo, int up);
 template <typename R>
 size_t sep3d_scratch_bytes(const Box3& b, int narr);
 
