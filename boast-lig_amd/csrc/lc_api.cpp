@@ -320,6 +320,22 @@ int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, 
     }
     return LC_OK;
   }
+  // single-pass 3D sym8 wavelet transform (symicomb8 runs as three sweeps)
+  if (d == 3 && kind != KCONV && id == WAV_SYM8 && !getenv("LIBCONV_DIRECT")) {
+    Box3 b;
+    for (int i = 0; i < 3; ++i) { b.n[i] = n[i]; b.bc[i] = bc[i]; b.nx[i] = nx[i]; b.ny[i] = ny[i]; }
+    hipError_t e = sepw3d<R>(kind, id, b, narr, ax.dev, ay.dev, (R)a, (R)a_y, st);
+    if (e != hipSuccess) return hip_fail(e, who);
+    if (a_x != 0.0) {
+      e = axpy((long)tout * narr, (R)a_x, ax.dev, ay.dev, st);
+      if (e != hipSuccess) return hip_fail(e, who);
+    }
+    if (!device_api) {
+      if (ay.host_in) HIPCHK(hipMemcpyAsync(y, ay.dev, (size_t)tout * narr * sizeof(R), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    return LC_OK;
+  }
   // intermediates: after sweep s the dims < s+1 carry ny
   int cur[16];
   for (int i = 0; i < d; ++i) cur[i] = nx[i];
@@ -792,6 +808,176 @@ int lc_wavelet_md(int prec, int wav, int inverse, int d, const int* n, const int
     return run_md<float>(kind, wav, d, n, bc, nx, ny, narr, (const float*)x, (float*)y, a, 0.0, a_y,
                          hs(stream), true);
   return fail(LC_ERR_ARG, "lc_wavelet_md: prec %d not in {4, 8}", prec);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Independent 3D operator applications grouped into stages: consecutive
+// items of the same kind and operator form one stage, executed as one
+// launch; stages run in item order on the stream.
+struct lc_batch {
+  int prec = 8;
+  int count = 0;
+  std::vector<BatchStage> stages;
+  hipStream_t cap = nullptr;     // private stream the graph is captured on
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename R>
+BatchItem<R> to_item(const lc_batch_item& u) {
+  BatchItem<R> it{};
+  for (int i = 0; i < 3; ++i) {
+    it.b.n[i] = u.n[i]; it.b.bc[i] = u.bc[i]; it.b.nx[i] = u.nx[i]; it.b.ny[i] = u.ny[i];
+    it.a[i] = u.a[i];
+  }
+  it.x = static_cast<const R*>(u.x);
+  it.y = static_cast<R*>(u.y);
+  it.a_x = u.a_x;
+  it.a_y = u.a_y;
+  it.dot = u.dot;
+  return it;
+}
+
+int check_item(const lc_batch_item& u, int k) {
+  const char* who = "lc_batch_create";
+  const bool wav = u.kind == LC_BATCH_DWT || u.kind == LC_BATCH_IDWT;
+  if (u.kind < LC_BATCH_SEP || u.kind > LC_BATCH_IDWT)
+    return fail(LC_ERR_ARG, "%s: item %d has kind %d", who, k, u.kind);
+  const int W = wav ? wav_window(u.op) : conv_window(u.op);
+  if (W < 0) return fail(LC_ERR_ARG, "%s: item %d has unknown operator %d", who, k, u.op);
+  for (int i = 0; i < 3; ++i) {
+    int rc = check_line(u.bc[i], u.n[i], u.nx[i], u.ny[i], W, who);
+    if (rc) return rc;
+  }
+  if (!u.x || !u.y || !is_device_ptr(u.x) || !is_device_ptr(u.y))
+    return fail(LC_ERR_ARG, "%s: item %d: x and y must be device pointers", who, k);
+  if (u.dot && (u.kind != LC_BATCH_SUM || !is_device_ptr(u.dot)))
+    return fail(LC_ERR_ARG, "%s: item %d: dot must be a device pointer of a SUM item", who, k);
+  if (u.kind != LC_BATCH_SUM && u.a_x != 0.0)
+    return fail(LC_ERR_UNSUPPORTED, "%s: item %d: a_x is supported for SUM items only", who, k);
+  if (wav && u.op != WAV_SYM8)
+    return fail(LC_ERR_UNSUPPORTED, "%s: item %d: batched wavelets support sym8 only", who, k);
+  return LC_OK;
+}
+
+// stage key: kind, operator and, for the symicomb8 wavelet (whose filter
+// centre differs between periodic and free lines), periodic vs free
+long stage_key(const lc_batch_item& u) {
+  const bool wav = u.kind == LC_BATCH_DWT || u.kind == LC_BATCH_IDWT;
+  const bool per = wav && u.op == WAV_SYMICOMB8 && u.bc[0] == 0;
+  return ((long)u.kind << 16) | ((long)u.op << 1) | (per ? 1 : 0);
+}
+
+template <typename R>
+int build_stages(lc_batch* b, int count, const lc_batch_item* items) {
+  int k = 0;
+  while (k < count) {
+    int e = k + 1;
+    while (e < count && stage_key(items[e]) == stage_key(items[k])) ++e;
+    std::vector<BatchItem<R>> its;
+    for (int i = k; i < e; ++i) its.push_back(to_item<R>(items[i]));
+    BatchStage st;
+    st.kind = items[k].kind;
+    st.id = items[k].op;
+    st.bc = items[k].bc[0];
+    st.count = e - k;
+    hipError_t er;
+    if (st.kind == LC_BATCH_SEP || st.kind == LC_BATCH_SUM)
+      er = fused_stage<R>(true, st.kind == LC_BATCH_SUM ? BK_SUM : BK_SEP, st.id, its.data(), st.count, &st,
+                          nullptr);
+    else
+      er = wav_stage<R>(true, st.kind == LC_BATCH_DWT ? KDWT : KIDWT, st.id, st.bc, its.data(), st.count,
+                        &st, nullptr);
+    b->stages.push_back(st);
+    if (er != hipSuccess) return hip_fail(er, "lc_batch_create");
+    k = e;
+  }
+  return LC_OK;
+}
+
+template <typename R>
+int launch_stages(lc_batch* b, hipStream_t st) {
+  for (BatchStage& s : b->stages) {
+    hipError_t er;
+    if (s.kind == LC_BATCH_SEP || s.kind == LC_BATCH_SUM)
+      er = fused_stage<R>(false, s.kind == LC_BATCH_SUM ? BK_SUM : BK_SEP, s.id, nullptr, 0, &s, st);
+    else
+      er = wav_stage<R>(false, s.kind == LC_BATCH_DWT ? KDWT : KIDWT, s.id, s.bc, nullptr, 0, &s, st);
+    if (er != hipSuccess) return hip_fail(er, "lc_batch_run");
+  }
+  return LC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lc_batch_create(lc_batch** plan, int prec, int count, const lc_batch_item* items) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_create: null plan");
+  *plan = nullptr;
+  if (prec != 4 && prec != 8) return fail(LC_ERR_ARG, "lc_batch_create: prec %d not in {4, 8}", prec);
+  if (count < 0 || (count > 0 && !items)) return fail(LC_ERR_ARG, "lc_batch_create: bad item list");
+  for (int k = 0; k < count; ++k) {
+    int rc = check_item(items[k], k);
+    if (rc) return rc;
+  }
+  lc_batch* b = new lc_batch;
+  b->prec = prec;
+  b->count = count;
+  int rc = prec == 8 ? build_stages<double>(b, count, items) : build_stages<float>(b, count, items);
+  if (rc) {
+    lc_batch_destroy(b);
+    return rc;
+  }
+  *plan = b;
+  return LC_OK;
+}
+
+int lc_batch_run(lc_batch* plan, void* stream) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_run: null plan");
+  return plan->prec == 8 ? launch_stages<double>(plan, hs(stream)) : launch_stages<float>(plan, hs(stream));
+}
+
+int lc_batch_run_graph(lc_batch* plan, void* stream) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_run_graph: null plan");
+  if (!plan->exec) {
+    if (!plan->cap) HIPCHK(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
+    HIPCHK(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
+    int rc = plan->prec == 8 ? launch_stages<double>(plan, plan->cap) : launch_stages<float>(plan, plan->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(plan->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "lc_batch_run_graph: capture");
+    plan->graph = g;
+    HIPCHK(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipGraphLaunch(plan->exec, hs(stream)));
+  return LC_OK;
+}
+
+int lc_batch_info(const lc_batch* plan, int* stages, long* blocks) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_info: null plan");
+  long nb = 0;
+  for (const BatchStage& s : plan->stages) nb += s.blocks;
+  if (stages) *stages = (int)plan->stages.size();
+  if (blocks) *blocks = nb;
+  return LC_OK;
+}
+
+void lc_batch_destroy(lc_batch* plan) {
+  if (!plan) return;
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  if (plan->cap) (void)hipStreamDestroy(plan->cap);
+  for (BatchStage& s : plan->stages) batch_stage_free(&s);
+  delete plan;
 }
 
 }  // extern "C"

@@ -24,56 +24,9 @@
 
 #include <algorithm>
 
-#include "lc_kernels.hpp"
+#include "lc_fused.hpp"
 
 namespace lc {
-
-struct FDim {   // one dim of a fused 3D operator
-  int n, bc;
-  int nx, ny;   // storage extents of input / output
-  int ox, oy;   // logical index of the first stored input / output element
-  int olo, nout;  // first logical output index and output count
-};
-
-__device__ __forceinline__ int fsrc(const FDim& g, int j) {
-  if (g.bc == PER) return pmod(j, g.n);
-  if (g.bc == GROW) return (j >= 0 && j < g.n) ? j : -1;
-  const int s = j - g.ox;
-  return (s >= 0 && s < g.nx) ? s : -1;  // beyond the stored input: never used
-}
-
-template <typename R>
-struct FusedArgs {
-  const R* x;
-  R* y;
-  FDim d0, d1, d2;
-  long nti, nto;
-  int narr;
-  int tiles_x, tiles_y, zchunk;  // zchunk = outputs per block along z
-  R a0, a1, a2, a_x, a_y;
-  int flags;
-  double* part;  // 3 sets of per-block energies (sum3d) or null
-  long nparts;   // blocks per array = sets stride
-};
-
-// Block -> (tile, z chunk, array).  Workgroups are dispatched round-robin
-// over the 8 XCDs of the chip, each with its own L2: the linear block id is
-// permuted so that every XCD receives one contiguous run of tiles, and the
-// filter halos that neighbouring tiles share are fetched once per XCD.
-__device__ __forceinline__ void block_coords(int tiles_x, int& bx, int& by, int& zc, int& arr) {
-  const int tiles = gridDim.x, nz = gridDim.y;
-  const int nb = tiles * nz * gridDim.z;
-  const int lin = blockIdx.x + tiles * (blockIdx.y + nz * blockIdx.z);
-  const int xcd = lin & 7, k = lin >> 3;
-  const int per = nb >> 3, rem = nb & 7;
-  const int nl = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
-  const int tile = nl % tiles;
-  const int rest = nl / tiles;
-  zc = rest % nz;
-  arr = rest / nz;
-  bx = tile % tiles_x;
-  by = tile / tiles_x;
-}
 
 template <typename R>
 __device__ __forceinline__ double wave_block_sum(double v, double* red, int nthreads) {
@@ -91,7 +44,8 @@ __device__ __forceinline__ double wave_block_sum(double v, double* red, int nthr
 
 // ====================================================================== sum3d
 template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD>
-__global__ __launch_bounds__(TX * TY / CPT) void sum3d_fused(FusedArgs<R> p) {
+__device__ __forceinline__ void sum3d_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
+                                           long bid) {
   constexpr int NT = TX * TY / CPT;
   constexpr int W = F::N, HL = -F::LO, HR = F::UP;
   constexpr int TW = (TX + HL + HR) | 1;    // LDS tile row stride (odd: no bank conflicts)
@@ -107,8 +61,6 @@ __global__ __launch_bounds__(TX * TY / CPT) void sum3d_fused(FusedArgs<R> p) {
   __shared__ double red[NT / 64 + 1];
 
   const int t = threadIdx.x;
-  int bx, by, zc, arr;
-  block_coords(p.tiles_x, bx, by, zc, arr);
   const R* __restrict__ xa = p.x + arr * p.nti;
   R* __restrict__ ya = p.y + arr * p.nto;
   const int X0 = p.d0.olo + bx * TX, Y0 = p.d1.olo + by * TY;
@@ -278,7 +230,6 @@ __global__ __launch_bounds__(TX * TY / CPT) void sum3d_fused(FusedArgs<R> p) {
     }
   }
   if (p.part) {
-    const long bid = (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc;
     double s = wave_block_sum<R>(d0, red, NT);
     if (t == 0 && arr == p.narr - 1) p.part[bid] = s;
     s = wave_block_sum<R>(d1, red, NT);
@@ -290,7 +241,7 @@ __global__ __launch_bounds__(TX * TY / CPT) void sum3d_fused(FusedArgs<R> p) {
 
 // ====================================================================== sep3d
 template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD>
-__global__ __launch_bounds__(TX * TY / CPT) void sep3d_fused(FusedArgs<R> p) {
+__device__ __forceinline__ void sep3d_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr) {
   constexpr int NT = TX * TY / CPT;
   constexpr int W = F::N;
   constexpr int TW = TX + W - 1;            // input tile row length
@@ -306,8 +257,6 @@ __global__ __launch_bounds__(TX * TY / CPT) void sep3d_fused(FusedArgs<R> p) {
   __shared__ R B[TH * BW];
 
   const int t = threadIdx.x;
-  int bx, by, zc, arr;
-  block_coords(p.tiles_x, bx, by, zc, arr);
   const R* __restrict__ xa = p.x + arr * p.nti;
   R* __restrict__ ya = p.y + arr * p.nto;
   const int X0 = p.d0.olo + bx * TX, Y0 = p.d1.olo + by * TY;
@@ -425,39 +374,48 @@ __global__ __launch_bounds__(TX * TY / CPT) void sep3d_fused(FusedArgs<R> p) {
   }
 }
 
+// ====================================================================== kernels
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD>
+__global__ __launch_bounds__(TX * TY / CPT) void sum3d_fused(FusedArgs<R> p) {
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sum3d_body<R, F, TX, TY, CPT, QX, BZ, HD>(p, bx, by, zc, arr,
+                                           (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD>
+__global__ __launch_bounds__(TX * TY / CPT) void sep3d_fused(FusedArgs<R> p) {
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sep3d_body<R, F, TX, TY, CPT, QX, BZ, AD>(p, bx, by, zc, arr);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD>
+__global__ __launch_bounds__(TX * TY / CPT) void sum3d_batch(const FusedArgs<R>* __restrict__ tab,
+                                                              const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  const int bx = tile % p.tiles_x, by = tile / p.tiles_x;
+  sum3d_body<R, F, TX, TY, CPT, QX, BZ, HD>(p, bx, by, zc, 0, (long)local);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD>
+__global__ __launch_bounds__(TX * TY / CPT) void sep3d_batch(const FusedArgs<R>* __restrict__ tab,
+                                                              const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  sep3d_body<R, F, TX, TY, CPT, QX, BZ, AD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0);
+}
+
 // ====================================================================== launch
-static FDim fdim(int n, int bc, int nx, int ny, int lo, int up) {
-  FDim g;
-  g.n = n; g.bc = bc; g.nx = nx; g.ny = ny;
-  g.ox = bc == SHRINK ? lo : 0;
-  g.oy = bc == GROW ? -up : 0;
-  g.olo = bc == GROW ? -up : 0;
-  g.nout = bc == GROW ? n + (up - lo) : n;
-  return g;
-}
-
-// z chunk: enough blocks for ~2 per CU, chunks no shorter than 4 * filter length
-static int pick_zchunk(long tiles, int nz, int narr, int w, int ncu) {
-  const long want = 2L * ncu;
-  long chunks = (want + tiles * narr - 1) / (tiles * narr);
-  const long maxc = std::max(1, nz / (4 * w));
-  chunks = std::min(chunks, maxc);
-  chunks = std::max(chunks, 1L);
-  return (int)((nz + chunks - 1) / chunks);
-}
-
-static int g_ncu = 0;
-static int num_cu() {
-  if (!g_ncu) {
-    int dev = 0;
-    hipDeviceProp_t pr;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess)
-      g_ncu = pr.multiProcessorCount;
-    if (g_ncu <= 0) g_ncu = 256;
-  }
-  return g_ncu;
-}
-
 template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD>
 static hipError_t launch_sum3d(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
                                R a_y, double* dot, hipStream_t st, double* scratch,
@@ -590,6 +548,138 @@ hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R 
   }
   return hipErrorInvalidValue;
 }
+
+// ====================================================================== batches
+hipError_t batch_upload(const void* tab, size_t tab_bytes, const int* first, int count, BatchStage* s) {
+  hipError_t e = hipMalloc(&s->tab, std::max<size_t>(tab_bytes, 1));
+  if (e != hipSuccess) return e;
+  e = hipMalloc(reinterpret_cast<void**>(&s->first), sizeof(int) * (count + 1));
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(s->tab, tab, tab_bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  return hipMemcpy(s->first, first, sizeof(int) * (count + 1), hipMemcpyHostToDevice);
+}
+
+void batch_stage_free(BatchStage* s) {
+  if (s->tab) (void)hipFree(s->tab);
+  if (s->first) (void)hipFree(s->first);
+  if (s->part) (void)hipFree(s->part);
+  if (s->dots) (void)hipFree(s->dots);
+  *s = BatchStage{};
+}
+
+// energies of item blockIdx.x: 3 sets of its blocks' partials, fixed order
+__global__ __launch_bounds__(256) void batch_dots(const double* __restrict__ part,
+                                                  const int* __restrict__ first,
+                                                  double* const* __restrict__ dots) {
+  __shared__ double red[4];
+  const int o = blockIdx.x;
+  double* dst = dots[o];
+  if (!dst) return;  // uniform per block
+  const long b0 = first[o], nb = first[o + 1] - first[o];
+  const double* pp = part + 3 * b0;
+  for (int k = 0; k < 3; ++k) {
+    double v = 0.0;
+    for (long i = threadIdx.x; i < nb; i += blockDim.x) v += pp[k * nb + i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) dst[k] = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+  }
+}
+
+// tile shapes of the batched kernels (the default single-call variant)
+template <typename R, class F>
+static hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count, BatchStage* s,
+                                hipStream_t st) {
+  constexpr int TX = 32, TY = 16, CPT = 2;
+  if (!build) {
+    if (s->blocks == 0) return hipSuccess;
+    const FusedArgs<R>* tab = static_cast<const FusedArgs<R>*>(s->tab);
+    if (kind == BK_SUM) {
+      sum3d_batch<R, F, TX, TY, CPT, 2, 4, 1><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess || !s->any_dot) return e;
+      batch_dots<<<s->count, 256, 0, st>>>(s->part, s->first, s->dots);
+    } else {
+      sep3d_batch<R, F, TX, TY, CPT, 4, 4, 1><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count);
+    }
+    return hipGetLastError();
+  }
+  std::vector<FusedArgs<R>> tab(count);
+  std::vector<int> first(count + 1, 0);
+  std::vector<double*> dots(count, nullptr);
+  long tiles_all = 0;
+  for (int o = 0; o < count; ++o) {
+    const Box3& b = it[o].b;
+    FusedArgs<R>& p = tab[o];
+    p = FusedArgs<R>{};
+    p.x = it[o].x; p.y = it[o].y;
+    p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+    p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+    p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+    p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+    p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+    p.narr = 1;
+    p.tiles_x = (p.d0.nout + TX - 1) / TX;
+    p.tiles_y = (p.d1.nout + TY - 1) / TY;
+    p.a0 = (R)it[o].a[0]; p.a1 = (R)it[o].a[1]; p.a2 = (R)it[o].a[2];
+    p.a_x = (R)it[o].a_x; p.a_y = (R)it[o].a_y;
+    if (kind == BK_SUM)
+      p.flags = (p.a_x != R(0) ? F_AX : 0) | (p.a_y != R(0) ? F_AY : 0) | (it[o].dot ? F_DOT : 0);
+    else
+      p.flags = (p.a0 != R(1) ? F_A : 0) | (p.a_y != R(0) ? F_AY : 0);
+    dots[o] = it[o].dot;
+    if (it[o].dot) s->any_dot = true;
+    tiles_all += (long)p.tiles_x * p.tiles_y;
+  }
+  // the same number of z chunks for every item: enough blocks for ~2 per CU
+  const long chunks = std::max(1L, (2L * num_cu() + tiles_all - 1) / std::max(1L, tiles_all));
+  long total = 0;
+  for (int o = 0; o < count; ++o) {
+    FusedArgs<R>& p = tab[o];
+    const long c = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / (4 * F::N))));
+    p.zchunk = std::max(1, (int)((p.d2.nout + c - 1) / c));
+    const long nb = (long)p.tiles_x * p.tiles_y * ((p.d2.nout + p.zchunk - 1) / p.zchunk);
+    first[o] = (int)total;
+    p.nparts = nb;
+    total += nb;
+  }
+  first[count] = (int)total;
+  if (total > 0x7fffffffL) return hipErrorInvalidValue;
+  s->blocks = (int)total;
+  if (kind == BK_SUM && s->any_dot) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->part), sizeof(double) * 3 * std::max(total, 1L));
+    if (e != hipSuccess) return e;
+    for (int o = 0; o < count; ++o)
+      if (it[o].dot) tab[o].part = s->part + 3L * first[o];
+    e = hipMalloc(reinterpret_cast<void**>(&s->dots), sizeof(double*) * count);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(s->dots, dots.data(), sizeof(double*) * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  return batch_upload(tab.data(), sizeof(FusedArgs<R>) * count, first.data(), count, s);
+}
+
+template <typename R>
+hipError_t fused_stage(bool build, int kind, int id, const BatchItem<R>* it, int count, BatchStage* s,
+                       hipStream_t st) {
+  if (kind != BK_SEP && kind != BK_SUM) return hipErrorInvalidValue;
+  switch (id) {
+    case OP_MD: return fused_stage_t<R, FMD>(build, kind, it, count, s, st);
+    case OP_IMD: return fused_stage_t<R, FIMD>(build, kind, it, count, s, st);
+    case OP_D2: return fused_stage_t<R, FD2>(build, kind, it, count, s, st);
+    case OP_D1: return fused_stage_t<R, FD1>(build, kind, it, count, s, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template hipError_t fused_stage<double>(bool, int, int, const BatchItem<double>*, int, BatchStage*,
+                                        hipStream_t);
+template hipError_t fused_stage<float>(bool, int, int, const BatchItem<float>*, int, BatchStage*,
+                                       hipStream_t);
 
 template hipError_t sum3d_fused_run<double>(int, const Box3&, int, const double*, double*, const double*,
                                             double, double, double*, hipStream_t, double*, size_t);

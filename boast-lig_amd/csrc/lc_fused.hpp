@@ -1,0 +1,113 @@
+// lc_fused.hpp -- shared pieces of the single-pass 3D kernels (internal).
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "lc_kernels.hpp"
+
+namespace lc {
+
+struct FDim {   // one dim of a fused 3D operator
+  int n, bc;
+  int nx, ny;   // storage extents of input / output
+  int ox, oy;   // logical index of the first stored input / output element
+  int olo, nout;  // first logical output index and output count
+};
+
+__device__ __forceinline__ int fsrc(const FDim& g, int j) {
+  if (g.bc == PER) return pmod(j, g.n);
+  if (g.bc == GROW) return (j >= 0 && j < g.n) ? j : -1;
+  const int s = j - g.ox;
+  return (s >= 0 && s < g.nx) ? s : -1;  // beyond the stored input: never used
+}
+
+template <typename R>
+struct FusedArgs {
+  const R* x;
+  R* y;
+  FDim d0, d1, d2;
+  long nti, nto;
+  int narr;
+  int tiles_x, tiles_y, zchunk;  // zchunk = outputs per block along z
+  R a0, a1, a2, a_x, a_y;
+  int flags;
+  double* part;  // 3 sets of per-block energies (sum3d) or null
+  long nparts;   // blocks per array = sets stride
+};
+
+// Block -> (tile, z chunk, array).  Workgroups are dispatched round-robin
+// over the 8 XCDs of the chip, each with its own L2: the linear block id is
+// permuted so that every XCD receives one contiguous run of tiles, and the
+// filter halos that neighbouring tiles share are fetched once per XCD.
+__device__ __forceinline__ void block_coords(int tiles_x, int& bx, int& by, int& zc, int& arr) {
+  const int tiles = gridDim.x, nz = gridDim.y;
+  const int nb = tiles * nz * gridDim.z;
+  const int lin = blockIdx.x + tiles * (blockIdx.y + nz * blockIdx.z);
+  const int xcd = lin & 7, k = lin >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  const int nl = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+  const int tile = nl % tiles;
+  const int rest = nl / tiles;
+  zc = rest % nz;
+  arr = rest / nz;
+  bx = tile % tiles_x;
+  by = tile / tiles_x;
+}
+
+// Batched forms: one launch over the blocks of `count` independent operator
+// applications.  first[o] is the first block of application o (first[count]
+// = total); the block finds its application by binary search, then its
+// (tile, z chunk) inside it.  Applications are ordered tiles-fastest, so the
+// XCD-contiguous order of block_coords keeps each application's tiles on
+// few XCDs.
+__device__ __forceinline__ int batch_item(const int* __restrict__ first, int count, int lin) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= lin) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int xcd_linear() {
+  const int nb = gridDim.x, lin = blockIdx.x;
+  const int xcd = lin & 7, k = lin >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+// ------------------------------------------------------------ host helpers
+inline FDim fdim(int n, int bc, int nx, int ny, int lo, int up) {
+  FDim g;
+  g.n = n; g.bc = bc; g.nx = nx; g.ny = ny;
+  g.ox = bc == SHRINK ? lo : 0;
+  g.oy = bc == GROW ? -up : 0;
+  g.olo = bc == GROW ? -up : 0;
+  g.nout = bc == GROW ? n + (up - lo) : n;
+  return g;
+}
+
+// z chunk: enough blocks for ~2 per CU, chunks no shorter than 4 * filter length
+inline int pick_zchunk(long tiles, int nz, int narr, int w, int ncu) {
+  const long want = 2L * ncu;
+  long chunks = (want + tiles * narr - 1) / (tiles * narr);
+  const long maxc = std::max(1, nz / (4 * w));
+  chunks = std::min(chunks, maxc);
+  chunks = std::max(chunks, 1L);
+  return (int)((nz + chunks - 1) / chunks);
+}
+
+inline int num_cu() {
+  static int g_ncu = 0;
+  if (!g_ncu) {
+    int dev = 0;
+    hipDeviceProp_t pr;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess)
+      g_ncu = pr.multiProcessorCount;
+    if (g_ncu <= 0) g_ncu = 256;
+  }
+  return g_ncu;
+}
+
+}  // namespace lc

@@ -84,4 +84,40 @@ size_t ws_sum3d_parts_bound(const Box3& b, int lo, int up);
 template <typename R>
 size_t sep3d_scratch_bytes(const Box3& b, int narr);
 
+// ------------------------------------------------------------ batches
+// A stage applies one operator to `count` independent 3D items in one
+// launch (lc_batch.cpp builds stages from the public batch plan).
+enum BatchKind : int { BK_SEP = 0, BK_SUM = 1, BK_DWT = 2, BK_IDWT = 3 };
+template <typename R>
+struct BatchItem {
+  Box3 b;
+  const R* x;
+  R* y;
+  double a[3];       // BK_SUM: a_d per dim; BK_SEP / wavelets: a[0] = a
+  double a_x, a_y;
+  double* dot;       // BK_SUM: device pointer to 3 energies, or null
+};
+struct BatchStage {
+  int kind = 0, id = 0, bc = 0, count = 0, blocks = 0;
+  void* tab = nullptr;       // device table of per-item kernel arguments
+  int* first = nullptr;      // device: first block of every item (count + 1)
+  double* part = nullptr;    // device: BK_SUM energy partials, 3 per block
+  double** dots = nullptr;   // device: BK_SUM energy destinations per item
+  bool any_dot = false;
+};
+// copies the argument table and block prefix to new device buffers
+hipError_t batch_upload(const void* tab, size_t tab_bytes, const int* first, int count, BatchStage* s);
+void batch_stage_free(BatchStage* s);
+// build (allocating, synchronous) or launch (stream-ordered, no host work)
+template <typename R>
+hipError_t fused_stage(bool build, int kind, int id, const BatchItem<R>* items, int count,
+                       BatchStage* s, hipStream_t st);
+template <typename R>
+hipError_t wav_stage(bool build, int kind, int id, int bc, const BatchItem<R>* items, int count,
+                     BatchStage* s, hipStream_t st);
+// single-pass 3D wavelet transform (kind KDWT / KIDWT), lc_fused_wav.hip
+template <typename R>
+hipError_t sepw3d(int kind, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                  hipStream_t st);
+
 }  // namespace lc

@@ -325,3 +325,87 @@ WAV_SYM8, WAV_SYMICOMB8 = 0, 1
 def extent(op, bc, n, side):
     """Input (side 0) / output (side 1) extent along a dim (lc_extent)."""
     return lib().lc_extent(op, bc, n, side)
+
+
+# ------------------------------------------------------------------ batches
+BATCH_SEP, BATCH_SUM, BATCH_DWT, BATCH_IDWT = 0, 1, 2, 3
+
+
+class _BatchItem(ctypes.Structure):
+    _fields_ = [('kind', _I), ('op', _I), ('n', _I * 3), ('bc', _I * 3), ('nx', _I * 3),
+                ('ny', _I * 3), ('x', _VP), ('y', _VP), ('a', _D * 3), ('a_x', _D), ('a_y', _D),
+                ('dot', _VP)]
+
+
+def _batch_lib():
+    L = lib()
+    if not getattr(L, '_batch_ready', False):
+        L.lc_batch_create.argtypes = [ctypes.POINTER(_VP), _I, _I, ctypes.POINTER(_BatchItem)]
+        L.lc_batch_create.restype = _I
+        L.lc_batch_run.argtypes = [_VP, _VP]
+        L.lc_batch_run.restype = _I
+        L.lc_batch_run_graph.argtypes = [_VP, _VP]
+        L.lc_batch_run_graph.restype = _I
+        L.lc_batch_info.argtypes = [_VP, _IP, ctypes.POINTER(ctypes.c_long)]
+        L.lc_batch_info.restype = _I
+        L.lc_batch_destroy.argtypes = [_VP]
+        L.lc_batch_destroy.restype = None
+        L._batch_ready = True
+    return L
+
+
+class Batch:
+    """A plan over many independent 3D operator applications (lc_batch_*).
+
+    items: dicts with keys kind (BATCH_*), op (FIL_* / WAV_*), n, bc, nx, ny
+    (3 ints each, bc may be one int), x, y (device tensors), and optionally
+    a (3 floats or one), a_x, a_y, dot (float64 device tensor of 3).  The
+    tensors must stay alive as long as the plan.
+    """
+
+    def __init__(self, items, prec=8):
+        L = _batch_lib()
+        arr = (_BatchItem * max(len(items), 1))()
+        self._keep = []
+        for k, it in enumerate(items):
+            b = arr[k]
+            b.kind, b.op = int(it['kind']), int(it['op'])
+            bc = it['bc']
+            bc = [bc] * 3 if isinstance(bc, (int, np.integer)) else list(bc)
+            for i in range(3):
+                b.n[i], b.bc[i] = int(it['n'][i]), int(bc[i])
+                b.nx[i], b.ny[i] = int(it['nx'][i]), int(it['ny'][i])
+            a = it.get('a', 1.0)
+            a = [a, 0.0, 0.0] if isinstance(a, (int, float)) else list(a)
+            for i in range(3):
+                b.a[i] = float(a[i])
+            b.a_x, b.a_y = float(it.get('a_x', 0.0)), float(it.get('a_y', 0.0))
+            b.x, b.y = it['x'].data_ptr(), it['y'].data_ptr()
+            dot = it.get('dot')
+            b.dot = dot.data_ptr() if dot is not None else None
+            self._keep.append(it)
+        h = _VP()
+        _rc(L.lc_batch_create(ctypes.byref(h), prec, len(items), arr))
+        self._h = h
+
+    def run(self, stream=None):
+        _rc(_batch_lib().lc_batch_run(self._h, _stream(stream)))
+
+    def run_graph(self, stream=None):
+        _rc(_batch_lib().lc_batch_run_graph(self._h, _stream(stream)))
+
+    def info(self):
+        s, b = _I(), ctypes.c_long()
+        _rc(_batch_lib().lc_batch_info(self._h, ctypes.byref(s), ctypes.byref(b)))
+        return {'stages': s.value, 'blocks': b.value}
+
+    def close(self):
+        if getattr(self, '_h', None):
+            _batch_lib().lc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

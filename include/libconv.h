@@ -222,6 +222,42 @@ int lc_conv_1ds(int prec, int fil, int d, const int *n, const int *bc, const int
 int lc_wavelet_md(int prec, int wav, int inverse, int d, const int *n, const int *bc,
                   const int *nx, const int *ny, int narr, const void *x, void *y, double a,
                   double a_y, void *stream);
+/* ------------------------------------------- batched 3D operators
+ * No counterpart in the reference, which applies each operator to one
+ * orbital at a time (BigDFT/GenericConvolution2.rb:2131-2245 composes one
+ * array): this groups many independent 3D applications -- orbitals of
+ * different sizes and boundary conditions -- into one launch per operator.
+ * Items with the same kind and operator that follow each other form one
+ * stage (one launch); stages run in item order, so a later stage may read
+ * what an earlier one wrote.  Items inside a stage must not overlap.
+ * All pointers are device pointers fixed at plan creation; running a plan
+ * does no host work beyond the launches and allocates nothing, so a plan
+ * can run inside a caller's stream capture.  lc_batch_run_graph captures
+ * the plan into a HIP graph on its first call and replays it afterwards.
+ *   SEP   y = a fil_2 fil_1 fil_0 x + a_y y          (a = a[0])
+ *   SUM   y = sum_d a[d] fil_d x + a_x x + a_y y,  dot[d] = <x, a[d] fil_d x>
+ *   DWT / IDWT  one 3D level of sym8 wavelet analysis / synthesis,
+ *               y = a t + a_y y (LC_WAV_SYM8 only)
+ * Sizes follow lc_conv_md / lc_wavelet_md (n, nx, ny per dim; wavelet
+ * arrays hold 2 nx / 2 ny values per dim).                                */
+enum { LC_BATCH_SEP = 0, LC_BATCH_SUM = 1, LC_BATCH_DWT = 2, LC_BATCH_IDWT = 3 };
+typedef struct lc_batch_item {
+  int kind;              /* LC_BATCH_* */
+  int op;                /* LC_FIL_* (SEP, SUM) or LC_WAV_* (DWT, IDWT) */
+  int n[3], bc[3], nx[3], ny[3];
+  const void *x;         /* device */
+  void *y;               /* device */
+  double a[3];           /* SUM: a[d]; others: a[0] = a */
+  double a_x, a_y;       /* a_x: SUM items only */
+  double *dot;           /* SUM: device pointer to 3 FP64 energies, or NULL */
+} lc_batch_item;
+typedef struct lc_batch lc_batch;
+int lc_batch_create(lc_batch **plan, int prec, int count, const lc_batch_item *items);
+int lc_batch_run(lc_batch *plan, void *stream);
+int lc_batch_run_graph(lc_batch *plan, void *stream);
+int lc_batch_info(const lc_batch *plan, int *stages, long *blocks);
+void lc_batch_destroy(lc_batch *plan);
+
 /* extent of the input (side 0) or output (side 1) along a dim for an
  * operator: op 0..3 = fil ids, 10 + wav = analysis, 20 + wav = synthesis */
 int lc_extent(int op, int bc, int n, int side);
