@@ -119,6 +119,138 @@ def run_gpu(torch, lc, w, dtype, steps, warmup):
                 ekin=dot.cpu().numpy(), x=x.double().cpu().numpy(), nsets=nsets)
 
 
+def run_dwt(torch, lc, dtype, steps, warmup, n=96):
+    """3D sym8 DWT then IDWT on n^3 pairs ((2n)^3 values), periodic: the
+    round trip of the wavelet configuration.  Returns per-op times and the
+    round-trip error against the input."""
+    g = torch.Generator(device='cuda').manual_seed(3)
+    tot = (2 * n) ** 3
+    s = torch.rand(tot, dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+    c = torch.empty_like(s)
+    back = torch.empty_like(s)
+    nn, bc = [n] * 3, [0] * 3
+    st = torch.cuda.current_stream()
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(st)
+        lc.wavelet_md(lc.WAV_SYM8, False, nn, bc, nn, nn, 1, s, c, stream=st)
+        if ev:
+            ev[1].record(st)
+        lc.wavelet_md(lc.WAV_SYM8, True, nn, bc, nn, nn, 1, c, back, stream=st)
+        if ev:
+            ev[2].record(st)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for i in range(steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    t_f = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
+    t_i = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
+    err = ((back.double() - s.double()).abs().max() / s.double().abs().max()).item()
+    sz = 4 if dtype == torch.float32 else 8
+    op_bytes = 2 * tot * sz
+    return {'pairs': nn, 'values': tot, 'dwt_ms': round(t_f, 5), 'idwt_ms': round(t_i, 5),
+            'GB_s': round(2 * op_bytes / ((t_f + t_i) * 1e-3) / 1e9, 1),
+            'frac_hbm': round(2 * op_bytes / ((t_f + t_i) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            'algorithmic_bytes': 2 * op_bytes, 'round_trip_rel_err': err}
+
+
+def batch_orbitals(count=512, seed=5):
+    """Orbital sizes ~U{24..96} per dim and BC ~U{periodic, grow, shrink}."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [([int(v) for v in rng.integers(24, 97, 3)], int(rng.choice([0, 1, -1])))
+            for _ in range(count)]
+
+
+def batch_items(torch, lc, orbs, dtype):
+    """Per orbital: MF3D -> K3D (energies) on its grid, and one sym8 DWT of a
+    (2 s)^3 array (the orbital's sizes in pairs)."""
+    a = [-0.5 / H ** 2] * 3
+    dots = torch.zeros((len(orbs), 3), dtype=torch.float64, device='cuda')
+    mf, kin, dwt, pts = [], [], [], 0
+    g = torch.Generator(device='cuda').manual_seed(55)
+    for k, (s, bc) in enumerate(orbs):
+        pts += s[0] * s[1] * s[2]
+        if bc == 0:
+            n_mf, xs, y1s, n_k, y2s = s, s, s, s, s
+        elif bc == 1:
+            y1s = [v + 15 for v in s]
+            n_mf, xs, n_k, y2s = s, s, y1s, [v + 28 for v in y1s]
+        else:
+            y1s = [v + 28 for v in s]
+            n_mf, xs, n_k, y2s = y1s, [v + 15 for v in y1s], s, s
+        pr = lambda v: v[0] * v[1] * v[2]
+        x = torch.rand(pr(xs), dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+        y1 = torch.empty(pr(y1s), dtype=dtype, device='cuda')
+        y2 = torch.empty(pr(y2s), dtype=dtype, device='cuda')
+        wi = [v + 7 if bc == -1 else v for v in s]
+        wo = [v + 7 if bc == 1 else v for v in s]
+        wx = torch.rand(8 * pr(wi), dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+        wy = torch.empty(8 * pr(wo), dtype=dtype, device='cuda')
+        mf.append(dict(kind=lc.BATCH_SEP, op=0, n=n_mf, bc=bc, nx=xs, ny=y1s, x=x, y=y1))
+        kin.append(dict(kind=lc.BATCH_SUM, op=2, n=n_k, bc=bc, nx=y1s, ny=y2s, x=y1, y=y2, a=a,
+                        dot=dots[k]))
+        dwt.append(dict(kind=lc.BATCH_DWT, op=0, n=s, bc=bc, nx=wi, ny=wo, x=wx, y=wy))
+    return mf + kin + dwt, pts, dots
+
+
+def run_batch(torch, lc, dtype, reps, count=512):
+    """512 orbitals: one graph replay of the batch plan vs one plain stream-API
+    call per orbital and operator (same kernels, same stream)."""
+    orbs = batch_orbitals(count)
+    items, pts, dots = batch_items(torch, lc, orbs, dtype)
+    prec = 4 if dtype == torch.float32 else 8
+    plan = lc.Batch(items, prec=prec)
+    st = torch.cuda.current_stream()
+    a = [-0.5 / H ** 2] * 3
+
+    def singles():
+        for it in items:
+            bc = [it['bc']] * 3
+            if it['kind'] == lc.BATCH_SEP:
+                lc.conv_md(it['op'], it['n'], bc, it['nx'], it['ny'], 1, it['x'], it['y'], stream=st)
+            elif it['kind'] == lc.BATCH_SUM:
+                lc.conv_1ds(it['op'], it['n'], bc, it['nx'], it['ny'], 1, a, 0.0, 0.0, it['x'], it['y'],
+                            dot=it['dot'], stream=st)
+            else:
+                lc.wavelet_md(it['op'], False, it['n'], bc, it['nx'], it['ny'], 1, it['x'], it['y'],
+                              stream=st)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, (time.perf_counter() - t0) * 1e3 / reps
+
+    t_single, w_single = timed(singles)
+    ref = [it['y'].clone() for it in items[::97]]
+    ref_dots = dots.clone()
+    t_graph, w_graph = timed(lambda: plan.run_graph(st))
+    same = all(torch.equal(r, it['y']) for r, it in zip(ref, items[::97]))
+    ddiff = ((dots - ref_dots).abs().max() / ref_dots.abs().max()).item()
+    info = plan.info()
+    plan.close()
+    return {'orbitals': count, 'nominal_points': pts, 'stages': info['stages'], 'blocks': info['blocks'],
+            'graph_ms': round(t_graph, 4), 'per_orbital_ms': round(t_single, 4),
+            'speedup': round(t_single / t_graph, 2),
+            'Gpts_s_graph': round(pts / (t_graph * 1e-3) / 1e9, 3),
+            'Gpts_s_per_orbital': round(pts / (t_single * 1e-3) / 1e9, 3),
+            'outputs_equal_sampled': bool(same), 'ekin_rel_diff': ddiff,
+            'sizes': 'per dim U{24..96}, BC U{periodic, grow, shrink}, seed 5; each orbital: '
+                     'MF3D -> K3D with energies, and a sym8 DWT of (2s)^3 values'}
+
+
 def errs(a, b):
     import numpy as np
     d = a - b
@@ -162,6 +294,7 @@ def main():
     ap.add_argument('--no-oracle', action='store_true', help='skip the FP64 oracle error check')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--fp64-only', action='store_true')
+    ap.add_argument('--no-extra', action='store_true', help='skip the wavelet and batch lines')
     ap.add_argument('--copy-calib', action='store_true',
                     help='only run a 1 GiB device copy (calibrates the HBM byte counters)')
     args = ap.parse_args()
@@ -228,6 +361,11 @@ def main():
                        'ratio_vs_fp64': round(g32 / gpts, 3),
                        'chain_GB_s': round(chain_bytes / 2 / (r32['ms'] * 1e-3) / 1e9, 1),
                        'mf_ms': round(r32['t_mf'], 5), 'kin_ms': round(r32['t_k'], 5)}
+    if not args.no_extra:
+        out['dwt3d'] = run_dwt(torch, lc, torch.float64, args.steps, args.warmup)
+        out['dwt3d']['fp32'] = run_dwt(torch, lc, torch.float32, args.steps, args.warmup)
+        out['batch'] = run_batch(torch, lc, torch.float64, reps=3)
+        out['batch']['fp32'] = run_batch(torch, lc, torch.float32, reps=3)
     if not args.no_oracle:
         import oracle as O
         y1r, y2r, dr = oracle_chain(O, w, r64['x'])
