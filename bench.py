@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, 'boast-lig_amd'))
 
 HBM_PEAK_GBS = 8000.0
+# HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE, scaled by
+# a plain copy's readings) for the kernel each chain operator launches
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'chain_256_shrink_hbm_traffic.txt')
 MALL_BYTES = 256 << 20
 H = 0.3
 
@@ -251,6 +254,18 @@ def run_batch(torch, lc, dtype, reps, count=512):
                      'MF3D -> K3D with energies, and a sym8 DWT of (2s)^3 values'}
 
 
+def measured_traffic(kernel):
+    """Bytes per launch of `kernel` from the committed counter summary, or None."""
+    import re
+    if not kernel or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        m = re.match(r'traffic\[(.*)\] = (\d+)', line.strip())
+        if m and m.group(1) == kernel:
+            return int(m.group(2))
+    return None
+
+
 def errs(a, b):
     import numpy as np
     d = a - b
@@ -332,6 +347,11 @@ def main():
     dom_t = max(r64['t_k'], r64['t_mf'])
     dom_b = k_bytes if dom == 'kinetic3d' else mf_bytes
     achieved = dom_b / (dom_t * 1e-3) / 1e9
+    mfa, kina = chain_args(w)
+    if dom == 'kinetic3d':
+        kname = lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, kina['n'], kina['bc'], kina['nx'], kina['ny'])
+    else:
+        kname = lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, mfa['n'], mfa['bc'], mfa['nx'], mfa['ny'])
     out = {
         'metric': 'Gpts/s & % of HBM roofline, 3D MagicFilter+Kinetic FP64 vs FP32 at 1e-6',
         'value': round(gpts, 4),
@@ -348,7 +368,8 @@ def main():
         'config': {'workload': w['desc'], 'grid_out': [w['ny2']] * 3, 'h': H, 'bc': w['bc'],
                    'buffer_sets': r64['nsets']},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None, 'kernel': dom,
+                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': measured_traffic(kname),
+                     'kernel': kname or dom,
                      'avg_ms': round(dom_t, 5), 'algorithmic_bytes': dom_b},
         'chain': {'GB_s': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9, 1),
                   'frac_hbm': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -819,6 +819,7 @@ int lc_wavelet_md(int prec, int wav, int inverse, int d, const int* n, const int
 struct lc_batch {
   int prec = 8;
   int count = 0;
+  std::vector<lc_batch_item> items;  // kept for re-building stages when tuning
   std::vector<BatchStage> stages;
   hipStream_t cap = nullptr;     // private stream the graph is captured on
   hipGraph_t graph = nullptr;
@@ -872,6 +873,36 @@ long stage_key(const lc_batch_item& u) {
   return ((long)u.kind << 16) | ((long)u.op << 1) | (per ? 1 : 0);
 }
 
+int item_form(const lc_batch_item& u) {
+  switch (u.kind) {
+    case LC_BATCH_SEP: return FORM_SEP;
+    case LC_BATCH_SUM: return FORM_SUM;
+    case LC_BATCH_DWT: return FORM_DWT;
+  }
+  return FORM_IDWT;
+}
+
+Box3 item_box(const lc_batch_item& u) {
+  Box3 b;
+  for (int i = 0; i < 3; ++i) { b.n[i] = u.n[i]; b.bc[i] = u.bc[i]; b.nx[i] = u.nx[i]; b.ny[i] = u.ny[i]; }
+  return b;
+}
+
+// variant of a new stage: the tuned one for its largest item's shape when it
+// has a batched form, else the default
+int stage_default_variant(const lc_batch_item* it, int count, int prec) {
+  int big = 0;
+  long most = -1;
+  for (int i = 0; i < count; ++i) {
+    const long p = (long)it[i].ny[0] * it[i].ny[1] * it[i].ny[2];
+    if (p > most) { most = p; big = i; }
+  }
+  const int form = item_form(it[big]);
+  const int v = select_variant(prec, form, it[big].op, item_box(it[big]));
+  if (v >= 0 && v < batch_variant_count(form)) return v;
+  return form == FORM_SUM || form == FORM_SEP ? 1 : 0;
+}
+
 template <typename R>
 int build_stages(lc_batch* b, int count, const lc_batch_item* items) {
   int k = 0;
@@ -885,6 +916,7 @@ int build_stages(lc_batch* b, int count, const lc_batch_item* items) {
     st.id = items[k].op;
     st.bc = items[k].bc[0];
     st.count = e - k;
+    st.variant = stage_default_variant(items + k, e - k, sizeof(R));
     hipError_t er;
     if (st.kind == LC_BATCH_SEP || st.kind == LC_BATCH_SUM)
       er = fused_stage<R>(true, st.kind == LC_BATCH_SUM ? BK_SUM : BK_SEP, st.id, its.data(), st.count, &st,
@@ -912,6 +944,79 @@ int launch_stages(lc_batch* b, hipStream_t st) {
   return LC_OK;
 }
 
+// (re)build stage s of plan b with kernel variant v
+template <typename R>
+hipError_t rebuild_stage(lc_batch* b, size_t si, int first_item, int v) {
+  BatchStage& st = b->stages[si];
+  BatchStage fresh;
+  fresh.kind = st.kind; fresh.id = st.id; fresh.bc = st.bc; fresh.count = st.count;
+  fresh.variant = v;
+  std::vector<BatchItem<R>> its;
+  for (int i = first_item; i < first_item + st.count; ++i) its.push_back(to_item<R>(b->items[i]));
+  hipError_t er;
+  if (st.kind == LC_BATCH_SEP || st.kind == LC_BATCH_SUM)
+    er = fused_stage<R>(true, st.kind == LC_BATCH_SUM ? BK_SUM : BK_SEP, st.id, its.data(), st.count,
+                        &fresh, nullptr);
+  else
+    er = wav_stage<R>(true, st.kind == LC_BATCH_DWT ? KDWT : KIDWT, st.id, st.bc, its.data(), st.count,
+                      &fresh, nullptr);
+  if (er != hipSuccess) {
+    batch_stage_free(&fresh);
+    return er;
+  }
+  batch_stage_free(&st);
+  st = fresh;
+  return hipSuccess;
+}
+
+template <typename R>
+hipError_t launch_one(BatchStage& s, hipStream_t st) {
+  if (s.kind == LC_BATCH_SEP || s.kind == LC_BATCH_SUM)
+    return fused_stage<R>(false, s.kind == LC_BATCH_SUM ? BK_SUM : BK_SEP, s.id, nullptr, 0, &s, st);
+  return wav_stage<R>(false, s.kind == LC_BATCH_DWT ? KDWT : KIDWT, s.id, s.bc, nullptr, 0, &s, st);
+}
+
+// every batched variant of every stage, min of `repeat` timed runs; keeps
+// the fastest per stage
+template <typename R>
+int tune_stages(lc_batch* b, int repeat, double* ms_out) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int first = 0;
+  int rc = LC_OK;
+  for (size_t si = 0; si < b->stages.size() && rc == LC_OK; ++si) {
+    const int form = item_form(b->items[first]);
+    int best = b->stages[si].variant;
+    double bt = 1e300;
+    for (int v = 0; v < batch_variant_count(form) && rc == LC_OK; ++v) {
+      hipError_t er = rebuild_stage<R>(b, si, first, v);
+      if (er != hipSuccess) { rc = hip_fail(er, "lc_batch_tune"); break; }
+      double tmin = 1e300;
+      for (int r = 0; r <= std::max(1, repeat); ++r) {  // first run untimed
+        er = hipEventRecord(e0, nullptr);
+        if (er == hipSuccess) er = launch_one<R>(b->stages[si], nullptr);
+        if (er == hipSuccess) er = hipEventRecord(e1, nullptr);
+        if (er == hipSuccess) er = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (er == hipSuccess) er = hipEventElapsedTime(&ms, e0, e1);
+        if (er != hipSuccess) { rc = hip_fail(er, "lc_batch_tune"); break; }
+        if (r > 0) tmin = std::min(tmin, (double)ms);
+      }
+      if (tmin < bt) { bt = tmin; best = v; }
+    }
+    if (rc == LC_OK && b->stages[si].variant != best) {
+      hipError_t er = rebuild_stage<R>(b, si, first, best);
+      if (er != hipSuccess) rc = hip_fail(er, "lc_batch_tune");
+    }
+    if (ms_out) ms_out[si] = bt;
+    first += b->stages[si].count;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -928,6 +1033,7 @@ int lc_batch_create(lc_batch** plan, int prec, int count, const lc_batch_item* i
   lc_batch* b = new lc_batch;
   b->prec = prec;
   b->count = count;
+  b->items.assign(items, items + count);
   int rc = prec == 8 ? build_stages<double>(b, count, items) : build_stages<float>(b, count, items);
   if (rc) {
     lc_batch_destroy(b);
@@ -968,6 +1074,90 @@ int lc_batch_info(const lc_batch* plan, int* stages, long* blocks) {
   for (const BatchStage& s : plan->stages) nb += s.blocks;
   if (stages) *stages = (int)plan->stages.size();
   if (blocks) *blocks = nb;
+  return LC_OK;
+}
+
+int lc_batch_tune(lc_batch* plan, int repeat, double* stage_ms) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_tune: null plan");
+  HIPCHK(hipDeviceSynchronize());
+  if (plan->exec) { (void)hipGraphExecDestroy(plan->exec); plan->exec = nullptr; }
+  if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
+  const int rc = plan->prec == 8 ? tune_stages<double>(plan, repeat, stage_ms)
+                                 : tune_stages<float>(plan, repeat, stage_ms);
+  if (rc) return rc;
+  HIPCHK(hipDeviceSynchronize());
+  return LC_OK;
+}
+
+int lc_batch_stage_variant(const lc_batch* plan, int stage) {
+  if (!plan || stage < 0 || stage >= (int)plan->stages.size())
+    return fail(LC_ERR_ARG, "lc_batch_stage_variant: bad plan or stage");
+  return plan->stages[stage].variant;
+}
+
+// ---------------------------------------------------------------- tuner
+static int form_check(int form, int op, const int* n, const int* bc, const int* nx, const int* ny,
+                      const char* who, Box3* b) {
+  if (form < LC_FORM_SEP || form > LC_FORM_IDWT) return fail(LC_ERR_ARG, "%s: form %d", who, form);
+  const bool wav = form == LC_FORM_DWT || form == LC_FORM_IDWT;
+  const int W = wav ? (op == WAV_SYM8 ? 8 : -1) : conv_window(op);
+  if (W < 0) return fail(LC_ERR_ARG, "%s: operator %d not tunable for form %d", who, op, form);
+  if (!n || !bc || !nx || !ny) return fail(LC_ERR_ARG, "%s: null size array", who);
+  for (int i = 0; i < 3; ++i) {
+    int rc = check_line(bc[i], n[i], nx[i], ny[i], W, who);
+    if (rc) return rc;
+    b->n[i] = n[i]; b->bc[i] = bc[i]; b->nx[i] = nx[i]; b->ny[i] = ny[i];
+  }
+  return LC_OK;
+}
+
+int lc_tune_md(int prec, int form, int op, const int* n, const int* bc, const int* nx, const int* ny,
+               int repeat, int* best, double* best_ms, double* times, int ntimes) {
+  Box3 b;
+  int rc = form_check(form, op, n, bc, nx, ny, "lc_tune_md", &b);
+  if (rc) return rc;
+  if (prec != 4 && prec != 8) return fail(LC_ERR_ARG, "lc_tune_md: prec %d not in {4, 8}", prec);
+  std::vector<double> t;
+  int bv = -1;
+  double bt = 0.0;
+  hipError_t e = prec == 8 ? tune_md<double>(form, op, b, repeat, &bv, &bt, &t)
+                           : tune_md<float>(form, op, b, repeat, &bv, &bt, &t);
+  if (e != hipSuccess) return hip_fail(e, "lc_tune_md");
+  if (best) *best = bv;
+  if (best_ms) *best_ms = bt;
+  for (int i = 0; times && i < ntimes && i < (int)t.size(); ++i) times[i] = t[i];
+  return LC_OK;
+}
+
+int lc_tune_save(const char* path) {
+  if (!path) return fail(LC_ERR_ARG, "lc_tune_save: null path");
+  const int n = tune_save(path);
+  return n < 0 ? fail(LC_ERR_ARG, "lc_tune_save: cannot write %s", path) : LC_OK;
+}
+
+int lc_tune_load(const char* path) {
+  if (!path) return fail(LC_ERR_ARG, "lc_tune_load: null path");
+  const int n = tune_load(path);
+  return n < 0 ? fail(LC_ERR_ARG, "lc_tune_load: cannot read %s", path) : LC_OK;
+}
+
+void lc_tune_clear(void) { tune_clear(); }
+
+int lc_variant_count(int form) {
+  if (form < LC_FORM_SEP || form > LC_FORM_IDWT) return fail(LC_ERR_ARG, "lc_variant_count: form %d", form);
+  return variant_count(form);
+}
+
+int lc_kernel_name(int prec, int form, int op, const int* n, const int* bc, const int* nx, const int* ny,
+                   char* buf, int len) {
+  Box3 b;
+  int rc = form_check(form, op, n, bc, nx, ny, "lc_kernel_name", &b);
+  if (rc) return rc;
+  if (!buf || len < 1) return fail(LC_ERR_ARG, "lc_kernel_name: no buffer");
+  const int v = select_variant(prec, form, op, b);
+  const std::string s = form == LC_FORM_DWT || form == LC_FORM_IDWT ? wav_kernel_name(prec, form, op, v, false)
+                                                                     : fused_kernel_name(prec, form, op, v, false);
+  snprintf(buf, (size_t)len, "%s", s.c_str());
   return LC_OK;
 }
 

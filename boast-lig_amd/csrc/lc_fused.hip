@@ -20,7 +20,10 @@
 // x~ is x wrapped along periodic dims and zero outside its box along free
 // dims, exactly the per-dim semantics of the 1D sweeps
 // (BigDFT/GenericConvolution2.rb:91-121 index ranges, :2160-2240 chains).
+#include <stdio.h>
 #include <stdlib.h>
+
+#include <string>
 
 #include <algorithm>
 
@@ -484,50 +487,120 @@ size_t fused_sum3d_parts(const Box3& b, int narr, int lo, int up, int w) {
   return (size_t)(tiles * maxc);
 }
 
-// Kernel variants (tile TX x TY, CPT columns per lane, QX outputs per
-// x-task, BZ planes per window slide, HD/AD prefetch depth).  The variant is
-// picked per call by the tuner (lc_tune.cpp); index 0 is the default.
-static int env_variant(const char* name) {
-  const char* e = getenv(name);
-  return e ? atoi(e) : -1;
-}
+// ====================================================================== variants
+// Compiled variants of the two fused forms: tile TX x TY, CPT rows per lane,
+// QX outputs per x-task, BZ planes per window slide, HD / AD prefetch depth.
+// Indices 0..4 have batched forms; 5..7 are the wave-specialised kernels of
+// lc_fused_ws.hip (single calls only).  The variant of a call comes from the
+// tuner's table for its exact shape, else LIBCONV_{SUM3D,SEP3D}_VARIANT,
+// else index 1 (fastest at 256^3 on MI355X).
+#define LC_SUM_VARIANTS(X)                                                    \
+  X(0, 32, 16, 2, 2, 4, 2) X(1, 32, 16, 2, 2, 4, 1) X(2, 32, 8, 2, 2, 4, 2)   \
+  X(3, 64, 8, 2, 4, 4, 2) X(4, 32, 16, 2, 2, 2, 2)
+#define LC_SEP_VARIANTS(X)                                                    \
+  X(0, 32, 16, 2, 4, 4, 4) X(1, 32, 16, 2, 4, 4, 1) X(2, 32, 8, 2, 4, 4, 2)   \
+  X(3, 64, 8, 2, 4, 4, 2) X(4, 32, 16, 2, 4, 4, 2)
+#define LC_WS_VARIANTS(X) X(5, 32, 8) X(6, 64, 8) X(7, 32, 16)
 
 template <typename R, class F>
 static hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
                                 R a_x, R a_y, double* dot, hipStream_t st, double* sc, size_t sb) {
   switch (v) {
-    case 1: return launch_sum3d<R, F, 32, 16, 2, 2, 4, 1>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
-    case 2: return launch_sum3d<R, F, 32, 8, 2, 2, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
-    case 3: return launch_sum3d<R, F, 64, 8, 2, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
-    case 4: return launch_sum3d<R, F, 32, 16, 2, 2, 2, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
-    case 0: return launch_sum3d<R, F, 32, 16, 2, 2, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
-    case 6: return launch_sum3d_ws<R, F, 64, 8, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
-    case 7: return launch_sum3d_ws<R, F, 32, 16, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
-    case 5: return launch_sum3d_ws<R, F, 32, 8, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
-    default: return launch_sum3d<R, F, 32, 16, 2, 2, 4, 1>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+#define X(I, TX, TY, CPT, QX, BZ, HD) \
+  case I: return launch_sum3d<R, F, TX, TY, CPT, QX, BZ, HD>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    LC_SUM_VARIANTS(X)
+#undef X
+#define X(I, TX, TY) \
+  case I: return launch_sum3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
+    LC_WS_VARIANTS(X)
+#undef X
   }
+  return hipErrorInvalidValue;
 }
 
 template <typename R, class F>
 static hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
                                 hipStream_t st) {
   switch (v) {
-    case 1: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 1>(b, narr, x, y, a, a_y, st);
-    case 2: return launch_sep3d<R, F, 32, 8, 2, 4, 4, 2>(b, narr, x, y, a, a_y, st);
-    case 3: return launch_sep3d<R, F, 64, 8, 2, 4, 4, 2>(b, narr, x, y, a, a_y, st);
-    case 4: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 2>(b, narr, x, y, a, a_y, st);
-    case 0: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 4>(b, narr, x, y, a, a_y, st);
-    case 6: return launch_sep3d_ws<R, F, 64, 8, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
-    case 7: return launch_sep3d_ws<R, F, 32, 16, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
-    case 5: return launch_sep3d_ws<R, F, 32, 8, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
-    default: return launch_sep3d<R, F, 32, 16, 2, 4, 4, 1>(b, narr, x, y, a, a_y, st);
+#define X(I, TX, TY, CPT, QX, BZ, AD) \
+  case I: return launch_sep3d<R, F, TX, TY, CPT, QX, BZ, AD>(b, narr, x, y, a, a_y, st);
+    LC_SEP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY) \
+  case I: return launch_sep3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
+    LC_WS_VARIANTS(X)
+#undef X
   }
+  return hipErrorInvalidValue;
+}
+
+int variant_count(int form) { return form == FORM_SUM || form == FORM_SEP ? 8 : wav_variant_count(); }
+int batch_variant_count(int form) { return form == FORM_SUM || form == FORM_SEP ? 5 : wav_variant_count(); }
+
+static const char* filter_tag(int id) {
+  switch (id) {
+    case OP_MD: return "FMD";
+    case OP_IMD: return "FIMD";
+    case OP_D2: return "FD2";
+    case OP_D1: return "FD1";
+  }
+  return "?";
+}
+
+std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
+  const char* r = prec == 4 ? "float" : "double";
+  char buf[160];
+  buf[0] = 0;
+  const bool sum = form == FORM_SUM;
+  switch (v) {
+#define X(I, TX, TY, CPT, QX, BZ, HD)                                                             \
+  case I:                                                                                        \
+    if (sum)                                                                                     \
+      snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>", batch ? "sum3d_batch" : \
+               "sum3d_fused", r, filter_tag(id), TX, TY, CPT, QX, BZ, HD);                        \
+    break;
+    LC_SUM_VARIANTS(X)
+#undef X
+#define X(I, TX, TY) \
+  case I: snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, 4, 4, 2>", sum ? "sum3d_ws" : "sep3d_ws", r, filter_tag(id), TX, TY); break;
+    LC_WS_VARIANTS(X)
+#undef X
+  }
+  if (!sum && v >= 0 && v < 5) {
+    switch (v) {
+#define X(I, TX, TY, CPT, QX, BZ, AD)                                                             \
+  case I:                                                                                        \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>", batch ? "sep3d_batch" :  \
+             "sep3d_fused", r, filter_tag(id), TX, TY, CPT, QX, BZ, AD);                          \
+    break;
+      LC_SEP_VARIANTS(X)
+#undef X
+    }
+  }
+  return buf;
+}
+
+static int env_variant(const char* name, int nvar) {
+  const char* e = getenv(name);
+  if (!e) return -1;
+  const int v = atoi(e);
+  return v >= 0 && v < nvar ? v : -1;
+}
+
+int select_variant(int prec, int form, int id, const Box3& b) {
+  int v = tuned_variant(prec, form, id, b);
+  if (v >= 0) return v;
+  if (form == FORM_SUM || form == FORM_SEP) {
+    v = env_variant(form == FORM_SUM ? "LIBCONV_SUM3D_VARIANT" : "LIBCONV_SEP3D_VARIANT", 8);
+    return v >= 0 ? v : 1;
+  }
+  v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
+  return v >= 0 ? v : 0;
 }
 
 template <typename R>
-hipError_t sum3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
-                           R a_y, double* dot, hipStream_t st, double* scratch, size_t sb) {
-  const int v = env_variant("LIBCONV_SUM3D_VARIANT");
+hipError_t sum3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
+                         R a_y, double* dot, hipStream_t st, double* scratch, size_t sb) {
   switch (id) {
     case OP_D2: return sum3d_variant<R, FD2>(v, b, narr, x, y, a, a_x, a_y, dot, st, scratch, sb);
     case OP_D1: return sum3d_variant<R, FD1>(v, b, narr, x, y, a, a_x, a_y, dot, st, scratch, sb);
@@ -538,8 +611,8 @@ hipError_t sum3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, co
 }
 
 template <typename R>
-hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
-  const int v = env_variant("LIBCONV_SEP3D_VARIANT");
+hipError_t sep3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                         hipStream_t st) {
   switch (id) {
     case OP_MD: return sep3d_variant<R, FMD>(v, b, narr, x, y, a, a_y, st);
     case OP_IMD: return sep3d_variant<R, FIMD>(v, b, narr, x, y, a, a_y, st);
@@ -547,6 +620,19 @@ hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R 
     case OP_D1: return sep3d_variant<R, FD1>(v, b, narr, x, y, a, a_y, st);
   }
   return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t sum3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
+                           R a_y, double* dot, hipStream_t st, double* scratch, size_t sb) {
+  const int v = select_variant(sizeof(R), FORM_SUM, id, b);
+  return sum3d_fused_v<R>(v, id, b, narr, x, y, a, a_x, a_y, dot, st, scratch, sb);
+}
+
+template <typename R>
+hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
+  const int v = select_variant(sizeof(R), FORM_SEP, id, b);
+  return sep3d_fused_v<R>(v, id, b, narr, x, y, a, a_y, st);
 }
 
 // ====================================================================== batches
@@ -590,24 +676,8 @@ __global__ __launch_bounds__(256) void batch_dots(const double* __restrict__ par
   }
 }
 
-// tile shapes of the batched kernels (the default single-call variant)
-template <typename R, class F>
-static hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count, BatchStage* s,
-                                hipStream_t st) {
-  constexpr int TX = 32, TY = 16, CPT = 2;
-  if (!build) {
-    if (s->blocks == 0) return hipSuccess;
-    const FusedArgs<R>* tab = static_cast<const FusedArgs<R>*>(s->tab);
-    if (kind == BK_SUM) {
-      sum3d_batch<R, F, TX, TY, CPT, 2, 4, 1><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess || !s->any_dot) return e;
-      batch_dots<<<s->count, 256, 0, st>>>(s->part, s->first, s->dots);
-    } else {
-      sep3d_batch<R, F, TX, TY, CPT, 4, 4, 1><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count);
-    }
-    return hipGetLastError();
-  }
+template <typename R, class F, int TX, int TY>
+static hipError_t fused_stage_build(int kind, const BatchItem<R>* it, int count, BatchStage* s) {
   std::vector<FusedArgs<R>> tab(count);
   std::vector<int> first(count + 1, 0);
   std::vector<double*> dots(count, nullptr);
@@ -663,6 +733,56 @@ static hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, in
   return batch_upload(tab.data(), sizeof(FusedArgs<R>) * count, first.data(), count, s);
 }
 
+template <typename R, class F>
+static hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count, BatchStage* s,
+                                hipStream_t st) {
+  if (build) {
+    if (kind == BK_SUM) {
+      switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, HD) \
+  case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
+        LC_SUM_VARIANTS(X)
+#undef X
+      }
+    } else {
+      switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, AD) \
+  case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
+        LC_SEP_VARIANTS(X)
+#undef X
+      }
+    }
+    return hipErrorInvalidValue;
+  }
+  if (s->blocks == 0) return hipSuccess;
+  const FusedArgs<R>* tab = static_cast<const FusedArgs<R>*>(s->tab);
+  if (kind == BK_SUM) {
+    switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, HD)                                                                   \
+  case I:                                                                                              \
+    sum3d_batch<R, F, TX, TY, CPT, QX, BZ, HD><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SUM_VARIANTS(X)
+#undef X
+      default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !s->any_dot) return e;
+    batch_dots<<<s->count, 256, 0, st>>>(s->part, s->first, s->dots);
+  } else {
+    switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, AD)                                                                   \
+  case I:                                                                                              \
+    sep3d_batch<R, F, TX, TY, CPT, QX, BZ, AD><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SEP_VARIANTS(X)
+#undef X
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
 template <typename R>
 hipError_t fused_stage(bool build, int kind, int id, const BatchItem<R>* it, int count, BatchStage* s,
                        hipStream_t st) {
@@ -689,5 +809,13 @@ template hipError_t sep3d_fused_run<double>(int, const Box3&, int, const double*
                                             hipStream_t);
 template hipError_t sep3d_fused_run<float>(int, const Box3&, int, const float*, float*, float, float,
                                            hipStream_t);
+template hipError_t sum3d_fused_v<double>(int, int, const Box3&, int, const double*, double*, const double*,
+                                          double, double, double*, hipStream_t, double*, size_t);
+template hipError_t sum3d_fused_v<float>(int, int, const Box3&, int, const float*, float*, const double*,
+                                         float, float, double*, hipStream_t, double*, size_t);
+template hipError_t sep3d_fused_v<double>(int, int, const Box3&, int, const double*, double*, double,
+                                          double, hipStream_t);
+template hipError_t sep3d_fused_v<float>(int, int, const Box3&, int, const float*, float*, float, float,
+                                         hipStream_t);
 
 }  // namespace lc

@@ -19,7 +19,10 @@
 //         step i at c + 2 (i - oy)
 // Periodic dims wrap, free dims read zero outside the stored input, exactly
 // as the 1D sweeps.  Epilogue: y = a * t + a_y * y.
+#include <stdio.h>
 #include <stdlib.h>
+
+#include <string>
 
 #include "lc_fused.hpp"
 
@@ -249,9 +252,33 @@ __global__ __launch_bounds__(2 * TX * TY) void sepw3d_batch(const FusedArgs<R>* 
 }
 
 // ====================================================================== host
-constexpr int WTX = 16, WTY = 8, WQX = 4, WBZ = 2, WAD = 2;
+// Compiled variants: tile TX x TY steps, QX outputs per x-task, BZ steps per
+// window slide, AD prefetch depth (2 TX TY lanes per block).
+#define LC_WAV_VARIANTS(X) \
+  X(0, 16, 8, 4, 2, 2) X(1, 8, 8, 4, 2, 2) X(2, 32, 4, 4, 2, 2) X(3, 16, 8, 4, 1, 1)
 
-template <typename R, class Op>
+int wav_variant_count() { return 4; }
+
+std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
+  (void)id;
+  const char* r = prec == 4 ? "float" : "double";
+  const char* op = form == FORM_IDWT ? "Idwt<lc::WSplit<lc::FLP, lc::FHP, 8, true> >"
+                                     : "Dwt<lc::WSplit<lc::FLP, lc::FHP, 8, false> >";
+  char buf[200];
+  buf[0] = 0;
+  switch (v) {
+#define X(I, TX, TY, QX, BZ, AD)                                                                   \
+  case I:                                                                                         \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d>", batch ? "sepw3d_batch" : "sepw3d_fused", \
+             r, op, TX, TY, QX, BZ, AD);                                                          \
+    break;
+    LC_WAV_VARIANTS(X)
+#undef X
+  }
+  return buf;
+}
+
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
 static FusedArgs<R> wav_args(const Box3& b, int narr, const R* x, R* y, R a, R a_y) {
   FusedArgs<R> p{};
   p.x = x; p.y = y;
@@ -261,16 +288,16 @@ static FusedArgs<R> wav_args(const Box3& b, int narr, const R* x, R* y, R a, R a
   p.nti = 8L * b.nx[0] * b.nx[1] * b.nx[2];
   p.nto = 8L * b.ny[0] * b.ny[1] * b.ny[2];
   p.narr = narr;
-  p.tiles_x = (p.d0.nout + WTX - 1) / WTX;
-  p.tiles_y = (p.d1.nout + WTY - 1) / WTY;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
   p.a0 = a; p.a_y = a_y;
   p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0);
   return p;
 }
 
-template <typename R, class Op>
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
 static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
-  FusedArgs<R> p = wav_args<R, Op>(b, narr, x, y, a, a_y);
+  FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, BZ, AD>(b, narr, x, y, a, a_y);
   const long tiles = (long)p.tiles_x * p.tiles_y;
   // z chunks no shorter than 2 W steps
   const long want = 2L * num_cu();
@@ -280,8 +307,20 @@ static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, 
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
-  sepw3d_fused<R, Op, WTX, WTY, WQX, WBZ, WAD><<<grid, 2 * WTX * WTY, 0, st>>>(p);
+  sepw3d_fused<R, Op, TX, TY, QX, BZ, AD><<<grid, 2 * TX * TY, 0, st>>>(p);
   return hipGetLastError();
+}
+
+template <typename R, class Op>
+static hipError_t sepw3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                                 hipStream_t st) {
+  switch (v) {
+#define X(I, TX, TY, QX, BZ, AD) \
+  case I: return launch_sepw3d<R, Op, TX, TY, QX, BZ, AD>(b, narr, x, y, a, a_y, st);
+    LC_WAV_VARIANTS(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
 }
 
 // The symicomb8 family is not instantiated: its single-pass kernel (420
@@ -290,22 +329,29 @@ static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, 
 // bounds and matches the oracle.  symicomb8 3D transforms run as three 1D
 // sweeps instead (lc_api.cpp).
 template <typename R>
-hipError_t sepw3d(int kind, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
-                  hipStream_t st) {
+hipError_t sepw3d_v(int v, int kind, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                    hipStream_t st) {
   if (id != WAV_SYM8) return hipErrorNotSupported;
-  if (kind == KDWT) return launch_sepw3d<R, Dwt<Sym8Dec>>(b, narr, x, y, a, a_y, st);
-  if (kind == KIDWT) return launch_sepw3d<R, Idwt<Sym8Rec>>(b, narr, x, y, a, a_y, st);
+  if (kind == KDWT) return sepw3d_variant<R, Dwt<Sym8Dec>>(v, b, narr, x, y, a, a_y, st);
+  if (kind == KIDWT) return sepw3d_variant<R, Idwt<Sym8Rec>>(v, b, narr, x, y, a, a_y, st);
   return hipErrorInvalidValue;
 }
 
+template <typename R>
+hipError_t sepw3d(int kind, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                  hipStream_t st) {
+  const int v = select_variant(sizeof(R), kind == KIDWT ? FORM_IDWT : FORM_DWT, id, b);
+  return sepw3d_v<R>(v, kind, id, b, narr, x, y, a, a_y, st);
+}
+
 // ---- batches of independent 3D sym8 wavelet transforms
-template <typename R, class Op>
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
 static hipError_t wav_stage_build(const BatchItem<R>* it, int count, BatchStage* s) {
   std::vector<FusedArgs<R>> tab(count);
   std::vector<int> first(count + 1, 0);
   long tiles_all = 0;
   for (int o = 0; o < count; ++o) {
-    tab[o] = wav_args<R, Op>(it[o].b, 1, it[o].x, it[o].y, (R)it[o].a[0], (R)it[o].a_y);
+    tab[o] = wav_args<R, Op, TX, TY, QX, BZ, AD>(it[o].b, 1, it[o].x, it[o].y, (R)it[o].a[0], (R)it[o].a_y);
     tiles_all += (long)tab[o].tiles_x * tab[o].tiles_y;
   }
   const long want = 2L * num_cu();
@@ -325,10 +371,27 @@ static hipError_t wav_stage_build(const BatchItem<R>* it, int count, BatchStage*
 }
 
 template <typename R, class Op>
-static hipError_t wav_stage_launch(const BatchStage& s, hipStream_t st) {
-  if (s.blocks == 0) return hipSuccess;
-  sepw3d_batch<R, Op, WTX, WTY, WQX, WBZ, WAD><<<s.blocks, 2 * WTX * WTY, 0, st>>>(
-      static_cast<const FusedArgs<R>*>(s.tab), s.first, s.count);
+static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, BatchStage* s, hipStream_t st) {
+  if (build) {
+    switch (s->variant) {
+#define X(I, TX, TY, QX, BZ, AD) \
+  case I: return wav_stage_build<R, Op, TX, TY, QX, BZ, AD>(it, count, s);
+      LC_WAV_VARIANTS(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+  }
+  if (s->blocks == 0) return hipSuccess;
+  const FusedArgs<R>* tab = static_cast<const FusedArgs<R>*>(s->tab);
+  switch (s->variant) {
+#define X(I, TX, TY, QX, BZ, AD)                                                                     \
+  case I:                                                                                           \
+    sepw3d_batch<R, Op, TX, TY, QX, BZ, AD><<<s->blocks, 2 * TX * TY, 0, st>>>(tab, s->first, s->count); \
+    break;
+    LC_WAV_VARIANTS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -337,10 +400,8 @@ hipError_t wav_stage(bool build, int kind, int id, int bc, const BatchItem<R>* i
                      BatchStage* s, hipStream_t st) {
   (void)bc;
   if (id != WAV_SYM8) return hipErrorNotSupported;
-  if (kind == KDWT)
-    return build ? wav_stage_build<R, Dwt<Sym8Dec>>(it, count, s) : wav_stage_launch<R, Dwt<Sym8Dec>>(*s, st);
-  if (kind == KIDWT)
-    return build ? wav_stage_build<R, Idwt<Sym8Rec>>(it, count, s) : wav_stage_launch<R, Idwt<Sym8Rec>>(*s, st);
+  if (kind == KDWT) return wav_stage_t<R, Dwt<Sym8Dec>>(build, it, count, s, st);
+  if (kind == KIDWT) return wav_stage_t<R, Idwt<Sym8Rec>>(build, it, count, s, st);
   return hipErrorInvalidValue;
 }
 
@@ -348,6 +409,10 @@ template hipError_t sepw3d<double>(int, int, const Box3&, int, const double*, do
                                    hipStream_t);
 template hipError_t sepw3d<float>(int, int, const Box3&, int, const float*, float*, float, float,
                                   hipStream_t);
+template hipError_t sepw3d_v<double>(int, int, int, const Box3&, int, const double*, double*, double, double,
+                                     hipStream_t);
+template hipError_t sepw3d_v<float>(int, int, int, const Box3&, int, const float*, float*, float, float,
+                                    hipStream_t);
 template hipError_t wav_stage<double>(bool, int, int, int, const BatchItem<double>*, int, BatchStage*,
                                       hipStream_t);
 template hipError_t wav_stage<float>(bool, int, int, int, const BatchItem<float>*, int, BatchStage*,

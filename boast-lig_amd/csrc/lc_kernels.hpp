@@ -1,6 +1,9 @@
 // lc_kernels.hpp -- launch interfaces of the libconv HIP kernels (internal).
 #pragma once
 
+#include <string>
+#include <vector>
+
 #include "lc_common.hpp"
 
 namespace lc {
@@ -99,6 +102,7 @@ struct BatchItem {
 };
 struct BatchStage {
   int kind = 0, id = 0, bc = 0, count = 0, blocks = 0;
+  int variant = 0;           // kernel variant of the stage (tuner index)
   void* tab = nullptr;       // device table of per-item kernel arguments
   int* first = nullptr;      // device: first block of every item (count + 1)
   double* part = nullptr;    // device: BK_SUM energy partials, 3 per block
@@ -119,5 +123,35 @@ hipError_t wav_stage(bool build, int kind, int id, int bc, const BatchItem<R>* i
 template <typename R>
 hipError_t sepw3d(int kind, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
                   hipStream_t st);
+
+// ------------------------------------------------------------ variants / tuner
+// Forms of the 3D operators the tuner selects kernels for.
+enum Form : int { FORM_SEP = 0, FORM_SUM = 1, FORM_DWT = 2, FORM_IDWT = 3 };
+int variant_count(int form);          // single-call variants
+int batch_variant_count(int form);    // variants 0..k-1 also have batched kernels
+int wav_variant_count();
+// variant a call of this shape uses: tuner table, else environment, else default
+int select_variant(int prec, int form, int id, const Box3& b);
+// tuner table lookup (lc_tune.cpp): -1 when the shape was not tuned
+int tuned_variant(int prec, int form, int id, const Box3& b);
+std::string fused_kernel_name(int prec, int form, int id, int v, bool batch);
+std::string wav_kernel_name(int prec, int form, int id, int v, bool batch);
+template <typename R>
+hipError_t sum3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
+                         R a_y, double* dot, hipStream_t st, double* scratch, size_t sb);
+template <typename R>
+hipError_t sep3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                         hipStream_t st);
+template <typename R>
+hipError_t sepw3d_v(int v, int kind, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                    hipStream_t st);
+
+// tuner (lc_tune.cpp)
+template <typename R>
+hipError_t tune_md(int form, int id, const Box3& b, int repeat, int* best, double* best_ms,
+                   std::vector<double>* times);
+int tune_save(const char* path);
+int tune_load(const char* path);
+void tune_clear();
 
 }  // namespace lc

@@ -350,6 +350,10 @@ def _batch_lib():
         L.lc_batch_info.restype = _I
         L.lc_batch_destroy.argtypes = [_VP]
         L.lc_batch_destroy.restype = None
+        L.lc_batch_tune.argtypes = [_VP, _I, _DP]
+        L.lc_batch_tune.restype = _I
+        L.lc_batch_stage_variant.argtypes = [_VP, _I]
+        L.lc_batch_stage_variant.restype = _I
         L._batch_ready = True
     return L
 
@@ -399,6 +403,19 @@ class Batch:
         _rc(_batch_lib().lc_batch_info(self._h, ctypes.byref(s), ctypes.byref(b)))
         return {'stages': s.value, 'blocks': b.value}
 
+    def tune(self, repeat=3):
+        """Tunes every stage over its kernel variants; returns the best ms per stage."""
+        n = self.info()['stages']
+        ms = (_D * max(n, 1))()
+        _rc(_batch_lib().lc_batch_tune(self._h, repeat, ms))
+        return [ms[i] for i in range(n)]
+
+    def stage_variant(self, stage):
+        v = _batch_lib().lc_batch_stage_variant(self._h, stage)
+        if v < 0:
+            check()
+        return v
+
     def close(self):
         if getattr(self, '_h', None):
             _batch_lib().lc_batch_destroy(self._h)
@@ -409,3 +426,71 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ autotuner
+FORM_SEP, FORM_SUM, FORM_DWT, FORM_IDWT = 0, 1, 2, 3
+
+
+def _tune_lib():
+    L = lib()
+    if not getattr(L, '_tune_ready', False):
+        L.lc_tune_md.argtypes = [_I, _I, _I, _IP, _IP, _IP, _IP, _I, _IP, _DP, _DP, _I]
+        L.lc_tune_md.restype = _I
+        L.lc_tune_save.argtypes = [ctypes.c_char_p]
+        L.lc_tune_save.restype = _I
+        L.lc_tune_load.argtypes = [ctypes.c_char_p]
+        L.lc_tune_load.restype = _I
+        L.lc_tune_clear.argtypes = []
+        L.lc_tune_clear.restype = None
+        L.lc_variant_count.argtypes = [_I]
+        L.lc_variant_count.restype = _I
+        L.lc_kernel_name.argtypes = [_I, _I, _I, _IP, _IP, _IP, _IP, ctypes.c_char_p, _I]
+        L.lc_kernel_name.restype = _I
+        L._tune_ready = True
+    return L
+
+
+def _dims3(n, bc, nx, ny):
+    bc = [bc] * 3 if isinstance(bc, (int, np.integer)) else bc
+    return [_ints(v) for v in (n, bc, nx, ny)]
+
+
+def tune_md(form, op, n, bc, nx, ny, repeat=3, prec=8):
+    """Times every kernel variant of a 3D form on random data of this shape,
+    records the fastest for the shape; returns {'best', 'best_ms', 'times'}."""
+    L = _tune_lib()
+    n_, b_, nx_, ny_ = _dims3(n, bc, nx, ny)
+    nv = L.lc_variant_count(form)
+    best, best_ms = _I(), _D()
+    times = (_D * nv)()
+    _rc(L.lc_tune_md(prec, form, op, n_.ctypes.data_as(_IP), b_.ctypes.data_as(_IP),
+                     nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), repeat, ctypes.byref(best),
+                     ctypes.byref(best_ms), times, nv))
+    return {'best': best.value, 'best_ms': best_ms.value, 'times': [times[i] for i in range(nv)]}
+
+
+def tune_save(path):
+    _rc(_tune_lib().lc_tune_save(os.fsencode(path)))
+
+
+def tune_load(path):
+    _rc(_tune_lib().lc_tune_load(os.fsencode(path)))
+
+
+def tune_clear():
+    _tune_lib().lc_tune_clear()
+
+
+def variant_count(form):
+    return _tune_lib().lc_variant_count(form)
+
+
+def kernel_name(prec, form, op, n, bc, nx, ny):
+    """Name of the kernel a call of this shape launches (as profilers show it)."""
+    L = _tune_lib()
+    n_, b_, nx_, ny_ = _dims3(n, bc, nx, ny)
+    buf = ctypes.create_string_buffer(256)
+    _rc(L.lc_kernel_name(prec, form, op, n_.ctypes.data_as(_IP), b_.ctypes.data_as(_IP),
+                         nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), buf, 256))
+    return buf.value.decode()

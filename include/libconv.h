@@ -257,6 +257,35 @@ int lc_batch_run(lc_batch *plan, void *stream);
 int lc_batch_run_graph(lc_batch *plan, void *stream);
 int lc_batch_info(const lc_batch *plan, int *stages, long *blocks);
 void lc_batch_destroy(lc_batch *plan);
+/* Tunes every stage of the plan over its kernel variants (min of `repeat`
+ * timed runs of the stage each, after one untimed run) and keeps the
+ * fastest; stage_ms (may be NULL) receives the best time per stage.  The
+ * runs write the plan's outputs (and accumulate where a_y != 0).           */
+int lc_batch_tune(lc_batch *plan, int repeat, double *stage_ms);
+int lc_batch_stage_variant(const lc_batch *plan, int stage);
+
+/* ------------------------------------------- autotuner
+ * In the spirit of BOAST's optimizer (GenericOptimization and
+ * ConvolutionOperator1d#optimize, BigDFT/GenericConvolution2.rb:744-837,
+ * :1223-1267): the compiled kernel variants of a 3D operator form (tile
+ * shape, outputs per lane, window step, prefetch depth) each run `repeat`
+ * times on random data of the given shape; the minimum time per variant is
+ * kept and the fastest variant is recorded for that exact shape, which later
+ * calls of the shape then use.  times[] (may be NULL) receives the min time
+ * of each variant (ms; infinity where a variant rejects the shape).  The
+ * table is saved and loaded as JSON; LIBCONV_TUNE_FILE names a file loaded
+ * at first use.  form: separable product (MagicFilter-style lc_conv_md), sum
+ * form (Kinetic-style lc_conv_1ds), sym8 DWT or IDWT (lc_wavelet_md).      */
+enum { LC_FORM_SEP = 0, LC_FORM_SUM = 1, LC_FORM_DWT = 2, LC_FORM_IDWT = 3 };
+int lc_tune_md(int prec, int form, int op, const int *n, const int *bc, const int *nx,
+               const int *ny, int repeat, int *best, double *best_ms, double *times, int ntimes);
+int lc_tune_save(const char *path);
+int lc_tune_load(const char *path);
+void lc_tune_clear(void);
+int lc_variant_count(int form);
+/* name of the kernel a call of this shape launches (as profilers show it) */
+int lc_kernel_name(int prec, int form, int op, const int *n, const int *bc, const int *nx,
+                   const int *ny, char *buf, int len);
 
 /* extent of the input (side 0) or output (side 1) along a dim for an
  * operator: op 0..3 = fil ids, 10 + wav = analysis, 20 + wav = synthesis */

@@ -1,0 +1,121 @@
+"""GPU: every compiled kernel variant against the FP64 oracle, the tuner's
+on-device search, and batch-plan tuning (outputs stay bit-identical to one
+call per item, since the batched variants differ only in tiling)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-13
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture()
+def lc(gpu):
+    import libconv
+    libconv.lib()
+    libconv.tune_clear()
+    yield libconv
+    libconv.tune_clear()
+
+
+@pytest.mark.parametrize('v', range(8))
+def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v):
+    torch = gpu
+    rng = np.random.default_rng(1200 + v)
+    n = [37, 21, 45]
+    bc = [1, -1, 0]
+    # MagicFilter, separable form
+    nx = [n[0], n[1] + 15, n[2]]
+    ny = [n[0] + 15, n[1], n[2]]
+    x = rng.uniform(-1, 1, int(np.prod(nx)))
+    ref = np.zeros(int(np.prod(ny)))
+    O.s0s0_md(O.MD, n, bc, nx, ny, 1, x, ref)
+    monkeypatch.setenv('LIBCONV_SEP3D_VARIANT', str(v))
+    monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', str(v))
+    y = torch.zeros(ref.size, dtype=torch.float64, device='cuda')
+    lc.conv_md(lc.FIL_MD, n, bc, nx, ny, 1, torch.from_numpy(x).cuda(), y)
+    torch.cuda.synchronize()
+    assert rel(y.cpu().numpy(), ref) < TOL64
+    # Kinetic, sum form with energies, a_x and a_y
+    nx = [n[0], n[1] + 28, n[2]]
+    ny = [n[0] + 28, n[1], n[2]]
+    x = rng.uniform(-1, 1, int(np.prod(nx)))
+    y0 = rng.uniform(-1, 1, int(np.prod(ny)))
+    a = [-5.5, -4.0, -3.0]
+    ref = y0.copy()
+    rdot = O.s0s0_1ds(O.D2, n, bc, nx, ny, 1, a, 0.0, 0.25, x, ref)
+    y = torch.from_numpy(y0).cuda()
+    dot = torch.zeros(3, dtype=torch.float64, device='cuda')
+    lc.conv_1ds(lc.FIL_D2, n, bc, nx, ny, 1, a, 0.0, 0.25, torch.from_numpy(x).cuda(), y, dot=dot)
+    torch.cuda.synchronize()
+    assert rel(y.cpu().numpy(), ref) < TOL64
+    assert np.abs(dot.cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
+
+
+@pytest.mark.parametrize('v', range(4))
+@pytest.mark.parametrize('inverse', [False, True])
+def test_every_wavelet_variant_matches_oracle(lc, gpu, monkeypatch, v, inverse):
+    torch = gpu
+    rng = np.random.default_rng(1300 + v)
+    n = [13, 17, 11]
+    bc = [0, 1, -1]
+    nx = [n[0], n[1], n[2] + 7]
+    ny = [n[0], n[1] + 7, n[2]]
+    x = rng.uniform(-1, 1, 8 * int(np.prod(nx)))
+    ref = np.zeros(8 * int(np.prod(ny)))
+    (O.s1s0_md if inverse else O.s0s1_md)(O.SYM8, n, bc, nx, ny, 1, x, ref)
+    monkeypatch.setenv('LIBCONV_WAV3D_VARIANT', str(v))
+    y = torch.zeros(ref.size, dtype=torch.float64, device='cuda')
+    lc.wavelet_md(lc.WAV_SYM8, inverse, n, bc, nx, ny, 1, torch.from_numpy(x).cuda(), y)
+    torch.cuda.synchronize()
+    assert rel(y.cpu().numpy(), ref) < TOL64
+
+
+def test_tune_md_records_fastest(lc, gpu):
+    shape = dict(n=[64, 48, 40], bc=0, nx=[64, 48, 40], ny=[64, 48, 40])
+    r = lc.tune_md(lc.FORM_SUM, lc.FIL_D2, repeat=2, **shape)
+    assert 0 <= r['best'] < 8 and np.isfinite(r['times'][:5]).all()
+    assert r['best_ms'] == min(r['times'])
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **shape) != ''
+    w = lc.tune_md(lc.FORM_IDWT, lc.WAV_SYM8, repeat=2, prec=4, **shape)
+    assert 0 <= w['best'] < 4 and np.isfinite(w['times']).all()
+
+
+def test_batch_tune_keeps_outputs(lc, gpu):
+    torch = gpu
+    rng = np.random.default_rng(1400)
+    items, ys = [], []
+    for k in range(6):
+        s = [int(v) for v in rng.integers(8, 40, 3)]
+        bc = int(rng.choice([0, 1, -1]))
+        nx = [v + 15 if bc == -1 else v for v in s]
+        ny = [v + 15 if bc == 1 else v for v in s]
+        x = torch.from_numpy(rng.uniform(-1, 1, int(np.prod(nx)))).cuda()
+        y = torch.zeros(int(np.prod(ny)), dtype=torch.float64, device='cuda')
+        items.append(dict(kind=lc.BATCH_SEP, op=lc.FIL_MD, n=s, bc=bc, nx=nx, ny=ny, x=x, y=y))
+        wnx = [v + 7 if bc == -1 else v for v in s]
+        wny = [v + 7 if bc == 1 else v for v in s]
+        wx = torch.from_numpy(rng.uniform(-1, 1, 8 * int(np.prod(wnx)))).cuda()
+        wy = torch.zeros(8 * int(np.prod(wny)), dtype=torch.float64, device='cuda')
+        items.append(dict(kind=lc.BATCH_DWT, op=lc.WAV_SYM8, n=s, bc=bc, nx=wnx, ny=wny, x=wx, y=wy))
+    items = items[0::2] + items[1::2]
+    plan = lc.Batch(items)
+    plan.run()
+    torch.cuda.synchronize()
+    before = [it['y'].clone() for it in items]
+    ms = plan.tune(repeat=2)
+    assert len(ms) == 2 and all(np.isfinite(ms))
+    assert 0 <= plan.stage_variant(0) < 5 and 0 <= plan.stage_variant(1) < 4
+    for it in items:
+        it['y'].fill_(3.0)
+    plan.run_graph()
+    torch.cuda.synchronize()
+    for b, it in zip(before, items):
+        assert torch.equal(b, it['y'])
