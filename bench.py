@@ -242,11 +242,17 @@ def run_batch(torch, lc, dtype, reps, count=512):
     t_graph, w_graph = timed(lambda: plan.run_graph(st))
     same = all(torch.equal(r, it['y']) for r, it in zip(ref, items[::97]))
     ddiff = ((dots - ref_dots).abs().max() / ref_dots.abs().max()).item()
+    t_pers, w_pers = timed(lambda: plan.run_persistent_graph(st))
+    pinfo = plan.persistent_info()
+    same_p = all(torch.equal(r, it['y']) for r, it in zip(ref, items[::97]))
     info = plan.info()
     plan.close()
     return {'orbitals': count, 'nominal_points': pts, 'stages': info['stages'], 'blocks': info['blocks'],
             'graph_ms': round(t_graph, 4), 'per_orbital_ms': round(t_single, 4),
             'speedup': round(t_single / t_graph, 2),
+            'persistent_graph_ms': round(t_pers, 4), 'persistent_speedup': round(t_single / t_pers, 2),
+            'persistent_grid': pinfo['grid'], 'persistent_status': pinfo['status'],
+            'persistent_outputs_equal_sampled': bool(same_p),
             'Gpts_s_graph': round(pts / (t_graph * 1e-3) / 1e9, 3),
             'Gpts_s_per_orbital': round(pts / (t_single * 1e-3) / 1e9, 3),
             'outputs_equal_sampled': bool(same), 'ekin_rel_diff': ddiff,

@@ -824,6 +824,9 @@ struct lc_batch {
   hipStream_t cap = nullptr;     // private stream the graph is captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  PersistPlan* persist = nullptr;  // one-launch form, built on first use
+  hipGraph_t pgraph = nullptr;
+  hipGraphExec_t pexec = nullptr;
 };
 
 namespace {
@@ -1077,6 +1080,84 @@ int lc_batch_info(const lc_batch* plan, int* stages, long* blocks) {
   return LC_OK;
 }
 
+static int persist_ready(lc_batch* plan) {
+  if (plan->persist) return LC_OK;
+  const int n = plan->count;
+  std::vector<int> kinds(n), ids(n);
+  for (int i = 0; i < n; ++i) {
+    const lc_batch_item& u = plan->items[i];
+    kinds[i] = u.kind == LC_BATCH_SEP ? BK_SEP : u.kind == LC_BATCH_SUM ? BK_SUM
+             : u.kind == LC_BATCH_DWT ? BK_DWT : BK_IDWT;
+    ids[i] = u.op;
+    const bool ok = (u.kind == LC_BATCH_SEP && (u.op == LC_FIL_MD || u.op == LC_FIL_IMD)) ||
+                    (u.kind == LC_BATCH_SUM && u.op == LC_FIL_D2) ||
+                    ((u.kind == LC_BATCH_DWT || u.kind == LC_BATCH_IDWT) && u.op == LC_WAV_SYM8);
+    if (!ok)
+      return fail(LC_ERR_UNSUPPORTED, "lc_batch_run_persistent: item %d (kind %d, operator %d) has no "
+                  "persistent form (SEP md/imd, SUM d2, sym8 DWT/IDWT)", i, u.kind, u.op);
+  }
+  hipError_t e;
+  if (plan->prec == 8) {
+    std::vector<BatchItem<double>> its;
+    for (int i = 0; i < n; ++i) its.push_back(to_item<double>(plan->items[i]));
+    e = persist_build<double>(its.data(), kinds.data(), ids.data(), n, &plan->persist);
+  } else {
+    std::vector<BatchItem<float>> its;
+    for (int i = 0; i < n; ++i) its.push_back(to_item<float>(plan->items[i]));
+    e = persist_build<float>(its.data(), kinds.data(), ids.data(), n, &plan->persist);
+  }
+  if (e != hipSuccess) return hip_fail(e, "lc_batch_run_persistent: build");
+  return LC_OK;
+}
+
+static int persist_spin_limit() {
+  const char* e = getenv("LIBCONV_PERSIST_SPIN_LIMIT");
+  const long v = e ? atol(e) : 0;
+  return v > 0 && v < 0x7fffffffL ? (int)v : (1 << 24);
+}
+
+int lc_batch_run_persistent(lc_batch* plan, void* stream) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_run_persistent: null plan");
+  int rc = persist_ready(plan);
+  if (rc) return rc;
+  hipError_t e = persist_launch(plan->persist, hs(stream), persist_spin_limit());
+  return e == hipSuccess ? LC_OK : hip_fail(e, "lc_batch_run_persistent");
+}
+
+int lc_batch_run_persistent_graph(lc_batch* plan, void* stream) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_run_persistent_graph: null plan");
+  int rc = persist_ready(plan);
+  if (rc) return rc;
+  if (!plan->pexec) {
+    if (!plan->cap) HIPCHK(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
+    HIPCHK(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
+    hipError_t le = persist_launch(plan->persist, plan->cap, persist_spin_limit());
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(plan->cap, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(le != hipSuccess ? le : e, "lc_batch_run_persistent_graph: capture");
+    }
+    plan->pgraph = g;
+    HIPCHK(hipGraphInstantiate(&plan->pexec, plan->pgraph, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipGraphLaunch(plan->pexec, hs(stream)));
+  return LC_OK;
+}
+
+int lc_batch_persistent_info(lc_batch* plan, int* grid, int* status) {
+  if (!plan) return fail(LC_ERR_ARG, "lc_batch_persistent_info: null plan");
+  int rc = persist_ready(plan);
+  if (rc) return rc;
+  if (grid) *grid = persist_grid(plan->persist);
+  if (status) {
+    HIPCHK(hipDeviceSynchronize());
+    hipError_t e = persist_status(plan->persist, status);
+    if (e != hipSuccess) return hip_fail(e, "lc_batch_persistent_info");
+  }
+  return LC_OK;
+}
+
 int lc_batch_tune(lc_batch* plan, int repeat, double* stage_ms) {
   if (!plan) return fail(LC_ERR_ARG, "lc_batch_tune: null plan");
   HIPCHK(hipDeviceSynchronize());
@@ -1165,7 +1246,10 @@ void lc_batch_destroy(lc_batch* plan) {
   if (!plan) return;
   if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
   if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  if (plan->pexec) (void)hipGraphExecDestroy(plan->pexec);
+  if (plan->pgraph) (void)hipGraphDestroy(plan->pgraph);
   if (plan->cap) (void)hipStreamDestroy(plan->cap);
+  if (plan->persist) persist_free(plan->persist);
   for (BatchStage& s : plan->stages) batch_stage_free(&s);
   delete plan;
 }

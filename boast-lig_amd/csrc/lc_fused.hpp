@@ -8,6 +8,9 @@
 
 namespace lc {
 
+// LDS carving: every array starts on a 16-byte boundary
+__host__ __device__ constexpr size_t lds_align(size_t b) { return (b + 15) & ~size_t(15); }
+
 struct FDim {   // one dim of a fused 3D operator
   int n, bc;
   int nx, ny;   // storage extents of input / output

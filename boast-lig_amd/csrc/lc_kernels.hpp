@@ -154,4 +154,15 @@ int tune_save(const char* path);
 int tune_load(const char* path);
 void tune_clear();
 
+// one persistent launch over a batch (lc_batch_persist.hip); kinds[] are
+// BatchKind values, ids[] the operators
+struct PersistPlan;
+template <typename R>
+hipError_t persist_build(const BatchItem<R>* items, const int* kinds, const int* ids, int count,
+                         PersistPlan** out);
+hipError_t persist_launch(const PersistPlan* pp, hipStream_t st, int spin_limit);
+hipError_t persist_status(const PersistPlan* pp, int* status);
+int persist_grid(const PersistPlan* pp);
+void persist_free(PersistPlan* pp);
+
 }  // namespace lc

@@ -350,6 +350,12 @@ def _batch_lib():
         L.lc_batch_info.restype = _I
         L.lc_batch_destroy.argtypes = [_VP]
         L.lc_batch_destroy.restype = None
+        L.lc_batch_run_persistent.argtypes = [_VP, _VP]
+        L.lc_batch_run_persistent.restype = _I
+        L.lc_batch_run_persistent_graph.argtypes = [_VP, _VP]
+        L.lc_batch_run_persistent_graph.restype = _I
+        L.lc_batch_persistent_info.argtypes = [_VP, _IP, _IP]
+        L.lc_batch_persistent_info.restype = _I
         L.lc_batch_tune.argtypes = [_VP, _I, _DP]
         L.lc_batch_tune.restype = _I
         L.lc_batch_stage_variant.argtypes = [_VP, _I]
@@ -402,6 +408,20 @@ class Batch:
         s, b = _I(), ctypes.c_long()
         _rc(_batch_lib().lc_batch_info(self._h, ctypes.byref(s), ctypes.byref(b)))
         return {'stages': s.value, 'blocks': b.value}
+
+    def run_persistent(self, stream=None):
+        """The whole plan in one persistent launch with device work queues."""
+        _rc(_batch_lib().lc_batch_run_persistent(self._h, _stream(stream)))
+
+    def run_persistent_graph(self, stream=None):
+        _rc(_batch_lib().lc_batch_run_persistent_graph(self._h, _stream(stream)))
+
+    def persistent_info(self):
+        """{'grid': blocks of the persistent launch, 'status': 0 unless a
+        dependency wait exceeded its bound (synchronises the device)}."""
+        g, st = _I(), _I()
+        _rc(_batch_lib().lc_batch_persistent_info(self._h, ctypes.byref(g), ctypes.byref(st)))
+        return {'grid': g.value, 'status': st.value}
 
     def tune(self, repeat=3):
         """Tunes every stage over its kernel variants; returns the best ms per stage."""

@@ -257,6 +257,20 @@ int lc_batch_run(lc_batch *plan, void *stream);
 int lc_batch_run_graph(lc_batch *plan, void *stream);
 int lc_batch_info(const lc_batch *plan, int *stages, long *blocks);
 void lc_batch_destroy(lc_batch *plan);
+/* The whole plan in ONE persistent launch (after a small reset kernel): a
+ * device-side work queue per XCD hands out (item, block) units, items wait
+ * on device counters for the items whose memory they depend on (found from
+ * the pointers and extents at plan creation), so stages overlap across
+ * items.  Items: SEP md/imd, SUM d2, sym8 DWT/IDWT.  Outputs equal
+ * lc_batch_run's bit for bit; energies are summed in another grouping.
+ * The _graph form captures the launch once and replays it.
+ * lc_batch_persistent_info reports the grid and (synchronising) the status
+ * word: non-zero when a dependency wait exceeded its bound
+ * (LIBCONV_PERSIST_SPIN_LIMIT sleeps, default 2^24), in which case the
+ * results of that run are not valid.                                        */
+int lc_batch_run_persistent(lc_batch *plan, void *stream);
+int lc_batch_run_persistent_graph(lc_batch *plan, void *stream);
+int lc_batch_persistent_info(lc_batch *plan, int *grid, int *status);
 /* Tunes every stage of the plan over its kernel variants (min of `repeat`
  * timed runs of the stage each, after one untimed run) and keeps the
  * fastest; stage_ms (may be NULL) receives the best time per stage.  The

@@ -232,3 +232,77 @@ def test_batch_rejects_bad_items(lc, gpu):
     w = dict(good, kind=2, op=1, n=[5, 5, 5], nx=[5, 5, 5], ny=[5, 5, 5])
     with pytest.raises(lc.LibconvError):
         lc.Batch([w])                                          # symicomb8 in a batch
+
+
+# ------------------------------------------------------------------ persistent
+def test_persistent_equals_stage_launches(lc, gpu):
+    """One persistent launch (device work queues, dependency counters)
+    gives the stage launches' outputs bit for bit; energies to 1e-13."""
+    torch = gpu
+    rng = np.random.default_rng(1500)
+    orbs = _orbitals(rng, 24, 5, 44)
+    items, bufs = _build(torch, orbs, rng, torch.float64)
+    plan = lc.Batch(items)
+    plan.run()
+    torch.cuda.synchronize()
+    ref = [[b.clone() for b in (y1, y2, dot, wy)] for (_, y1, y2, dot, _, wy) in bufs]
+    for (_, y1, y2, dot, _, wy) in bufs:
+        for b in (y1, y2, dot, wy):
+            b.fill_(5.0)
+    plan.run_persistent()
+    info = plan.persistent_info()
+    assert info['status'] == 0 and info['grid'] >= 8
+    for k, (_, y1, y2, dot, _, wy) in enumerate(bufs):
+        assert torch.equal(ref[k][0], y1), k
+        assert torch.equal(ref[k][1], y2), k
+        assert torch.equal(ref[k][3], wy), k
+        d_p, d_s = dot.cpu().numpy(), ref[k][2].cpu().numpy()
+        assert np.abs(d_p - d_s).max() <= 1e-13 * np.abs(d_s).max(), k
+
+
+@pytest.mark.parametrize('dtype_name', ['float64', 'float32'])
+def test_persistent_graph_replays(lc, gpu, dtype_name):
+    torch = gpu
+    dtype = getattr(torch, dtype_name)
+    rng = np.random.default_rng(1600)
+    orbs = _orbitals(rng, 16, 6, 36)
+    items, bufs = _build(torch, orbs, rng, dtype)
+    plan = lc.Batch(items, prec=8 if dtype_name == 'float64' else 4)
+    for rep in range(3):
+        for (x, _, _, _, wx, _) in bufs:
+            x.uniform_(-1, 1)
+            wx.uniform_(-1, 1)
+        plan.run_persistent_graph()
+        torch.cuda.synchronize()
+        got = [[b.clone() for b in (y1, y2, dot, wy)] for (_, y1, y2, dot, _, wy) in bufs]
+        plan.run()
+        torch.cuda.synchronize()
+        for k, (_, y1, y2, dot, _, wy) in enumerate(bufs):
+            assert torch.equal(got[k][0], y1), (rep, k)
+            assert torch.equal(got[k][1], y2), (rep, k)
+            assert torch.equal(got[k][3], wy), (rep, k)
+            assert np.abs(got[k][2].cpu().numpy() - dot.cpu().numpy()).max() <= \
+                1e-13 * np.abs(dot.cpu().numpy()).max(), (rep, k)
+    assert plan.persistent_info()['status'] == 0
+
+
+def test_persistent_bounded_wait_ends(lc, gpu, monkeypatch):
+    """With a wait bound of one sleep, dependent units give up at once: the
+    launch still ends, and the status word tells whether any wait did."""
+    torch = gpu
+    rng = np.random.default_rng(1700)
+    items, bufs = _build(torch, _orbitals(rng, 6, 8, 24), rng, torch.float64)
+    monkeypatch.setenv('LIBCONV_PERSIST_SPIN_LIMIT', '1')
+    plan = lc.Batch(items)
+    plan.run_persistent()
+    assert plan.persistent_info()['status'] in (0, 1)
+
+
+def test_persistent_rejects_items_without_a_persistent_form(lc, gpu):
+    torch = gpu
+    x = torch.zeros(1000, dtype=torch.float64, device='cuda')
+    y = torch.zeros(1000, dtype=torch.float64, device='cuda')
+    it = dict(kind=1, op=0, n=[10, 10, 10], bc=0, nx=[10, 10, 10], ny=[10, 10, 10], x=x, y=y)  # SUM md
+    plan = lc.Batch([it])
+    with pytest.raises(lc.LibconvError):
+        plan.run_persistent()
