@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void sum3d_direct(const R* __restrict__ x, R* 
 #pragma unroll
       for (int l = 0; l < F::N; ++l) {
         const int j = src_index(g0, i0 + F::LO + l);
-        if (j >= 0) t0 = __builtin_fma(row[j], tap<R, F>(l), t0);
+        if (j >= 0) t0 = fmar(row[j], tap<R, F>(l), t0);
       }
     }
     if (c0 >= 0 && c2 >= 0) {
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void sum3d_direct(const R* __restrict__ x, R* 
 #pragma unroll
       for (int l = 0; l < F::N; ++l) {
         const int j = src_index(g1, i1 + F::LO + l);
-        if (j >= 0) t1 = __builtin_fma(col[j * s1], tap<R, F>(l), t1);
+        if (j >= 0) t1 = fmar(col[j * s1], tap<R, F>(l), t1);
       }
     }
     if (c0 >= 0 && c1 >= 0) {
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void sum3d_direct(const R* __restrict__ x, R* 
 #pragma unroll
       for (int l = 0; l < F::N; ++l) {
         const int j = src_index(g2, i2 + F::LO + l);
-        if (j >= 0) t2 = __builtin_fma(col[j * s2], tap<R, F>(l), t2);
+        if (j >= 0) t2 = fmar(col[j * s2], tap<R, F>(l), t2);
       }
     }
     t0 = t0 * a0;
@@ -120,11 +120,11 @@ __global__ __launch_bounds__(256) void sum3d_direct(const R* __restrict__ x, R* 
       d2 = (double)xc * (double)t2;
     }
     R tot = t0;
-    if ((flags & F_AX) && has_x) tot = __builtin_fma(xc, a_x, tot);
+    if ((flags & F_AX) && has_x) tot = fmar(xc, a_x, tot);
     tot = t1 + tot;
     tot = t2 + tot;
     R* yp = ya + (i0 - g0.oy) + (long)(i1 - g1.oy) * g0.ny + (long)(i2 - g2.oy) * g0.ny * g1.ny;
-    if (flags & F_AY) tot = __builtin_fma(a_y, *yp, tot);
+    if (flags & F_AY) tot = fmar(a_y, *yp, tot);
     *yp = tot;
   }
   if (part) {

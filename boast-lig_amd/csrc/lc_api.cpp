@@ -251,11 +251,11 @@ int run_1d(int kind, int id, int d, int idim, const int* n, int bc, const int* n
 // ---------------------------------------------------------------- md core
 __global__ void axpy_kernel_d(long n, double a, const double* x, double* y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = __builtin_fma(x[i], a, y[i]);
+  if (i < n) y[i] = fmar(x[i], a, y[i]);
 }
 __global__ void axpy_kernel_f(long n, float a, const float* x, float* y) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = __builtin_fma(x[i], a, y[i]);
+  if (i < n) y[i] = fmar(x[i], a, y[i]);
 }
 hipError_t axpy(long n, double a, const double* x, double* y, hipStream_t st) {
   if (n > 0) axpy_kernel_d<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, x, y);
@@ -902,8 +902,9 @@ int stage_default_variant(const lc_batch_item* it, int count, int prec) {
   }
   const int form = item_form(it[big]);
   const int v = select_variant(prec, form, it[big].op, item_box(it[big]));
-  if (v >= 0 && v < batch_variant_count(form)) return v;
-  return form == FORM_SUM || form == FORM_SEP ? 1 : 0;
+  if (has_batch_form(form, v)) return v;
+  if (form == FORM_SUM) return prec == 4 ? 21 : 17;  // fastest batched kinetic forms
+  return form == FORM_SEP ? 12 : 0;
 }
 
 template <typename R>
@@ -992,7 +993,8 @@ int tune_stages(lc_batch* b, int repeat, double* ms_out) {
     const int form = item_form(b->items[first]);
     int best = b->stages[si].variant;
     double bt = 1e300;
-    for (int v = 0; v < batch_variant_count(form) && rc == LC_OK; ++v) {
+    for (int v = 0; v < variant_count(form) && rc == LC_OK; ++v) {
+      if (!has_batch_form(form, v)) continue;
       hipError_t er = rebuild_stage<R>(b, si, first, v);
       if (er != hipSuccess) { rc = hip_fail(er, "lc_batch_tune"); break; }
       double tmin = 1e300;

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "lc_body_conv.hpp"
+#include "lc_body_sum2.hpp"
 #include "lc_body_wav.hpp"
 
 namespace lc {
@@ -48,14 +49,14 @@ struct PState {  // device; zeroed before every run
 
 // tile shapes of the bodies in the persistent kernel (256 lanes each)
 #define LC_P_SEP(R, F) sep3d_body<R, F, 32, 16, 2, 4, 4, 1>
-#define LC_P_SUM(R) sum3d_body<R, FD2, 32, 16, 2, 2, 4, 1>
+#define LC_P_SUM(R) sum3d2_body<R, FD2, 16, 16, 4, 4, 2, 0>
 #define LC_P_WAV(R, OP) sepw3d_body<R, OP, 16, 8, 4, 2, 2>
 
 template <typename R>
 constexpr size_t persist_lds_bytes() {
   return std::max(std::max(sep3d_lds_bytes<R, FMD, 32, 16, 2, 4, 4, 1>(),
                            sep3d_lds_bytes<R, FIMD, 32, 16, 2, 4, 4, 1>()),
-                  std::max(sum3d_lds_bytes<R, FD2, 32, 16, 2, 2, 4, 1>(),
+                  std::max(Sum2Geo<R, FD2, 16, 16, 4, 4, 2>::lds_bytes(),
                            sepw3d_lds_bytes<R, Dwt<Sym8Dec>, 16, 8, 4, 2, 2>()));
 }
 
@@ -228,7 +229,7 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
       tab[i] = id == OP_MD ? conv_args<R, FMD, 32, 16>(it[i], false) : conv_args<R, FIMD, 32, 16>(it[i], false);
     } else if (k == BK_SUM && id == OP_D2) {
       pi.body = PB_SUM_D2;
-      tab[i] = conv_args<R, FD2, 32, 16>(it[i], true);
+      tab[i] = conv_args<R, FD2, 16, 16>(it[i], true);
       pi.dot = it[i].dot;
     } else if ((k == BK_DWT || k == BK_IDWT) && id == WAV_SYM8) {
       pi.body = k == BK_DWT ? PB_DWT : PB_IDWT;

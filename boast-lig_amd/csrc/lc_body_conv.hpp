@@ -156,7 +156,7 @@ __device__ __forceinline__ void sum3d_body(const FusedArgs<R>& p, int bx, int by
           for (int q = 0; q < QX; ++q) {
             R s = R(0);
 #pragma unroll
-            for (int k = 0; k < W; ++k) s = __builtin_fma(wv[q + k], tap<R, F>(k), s);
+            for (int k = 0; k < W; ++k) s = fmar(wv[q + k], tap<R, F>(k), s);
             SX[r * SXW + c0 + q] = s;
           }
         }
@@ -172,7 +172,7 @@ __device__ __forceinline__ void sum3d_body(const FusedArgs<R>& p, int bx, int by
           for (int c = 0; c < CPT; ++c) {
             R s = R(0);
 #pragma unroll
-            for (int k = 0; k < W; ++k) s = __builtin_fma(wv[c + k], tap<R, F>(k), s);
+            for (int k = 0; k < W; ++k) s = fmar(wv[c + k], tap<R, F>(k), s);
             yt[c] = s;
           }
         }
@@ -180,7 +180,7 @@ __device__ __forceinline__ void sum3d_body(const FusedArgs<R>& p, int bx, int by
         for (int c = 0; c < CPT; ++c) {
           R s = R(0);
 #pragma unroll
-          for (int k = 0; k < W; ++k) s = __builtin_fma(win[c][u + k], tap<R, F>(k), s);
+          for (int k = 0; k < W; ++k) s = fmar(win[c][u + k], tap<R, F>(k), s);
           zt[c] = s;
         }
         // ---- combine
@@ -193,17 +193,17 @@ __device__ __forceinline__ void sum3d_body(const FusedArgs<R>& p, int bx, int by
           const R t1 = yt[c] * p.a1;
           const R t2 = zt[c] * p.a2;
           if (want_dot && has_x) {
-            d0 = __builtin_fma((double)xc, (double)t0, d0);
-            d1 = __builtin_fma((double)xc, (double)t1, d1);
-            d2 = __builtin_fma((double)xc, (double)t2, d2);
+            d0 = fmar((double)xc, (double)t0, d0);
+            d1 = fmar((double)xc, (double)t1, d1);
+            d2 = fmar((double)xc, (double)t2, d2);
           }
-          if ((p.flags & F_AX) && has_x) t0 = __builtin_fma(xc, p.a_x, t0);
+          if ((p.flags & F_AX) && has_x) t0 = fmar(xc, p.a_x, t0);
           R tot = t1 + t0;
           tot = t2 + tot;
           if (cout_ok[c]) {
             R* yp = ya + (X0 + tx - p.d0.oy) +
                     (long)p.d0.ny * ((Y0 + ty0 + c - p.d1.oy) + (long)p.d1.ny * (z - p.d2.oy));
-            if (p.flags & F_AY) tot = __builtin_fma(p.a_y, *yp, tot);
+            if (p.flags & F_AY) tot = fmar(p.a_y, *yp, tot);
             *yp = tot;
           }
         }
@@ -305,7 +305,7 @@ __device__ __forceinline__ void sep3d_body(const FusedArgs<R>& p, int bx, int by
       for (int q = 0; q < QX; ++q) {
         R sum = R(0);
 #pragma unroll
-        for (int k = 0; k < W; ++k) sum = __builtin_fma(wv[q + k], tap<R, F>(k), sum);
+        for (int k = 0; k < W; ++k) sum = fmar(wv[q + k], tap<R, F>(k), sum);
         B[r * BW + c0 + q] = sum;
       }
     }
@@ -317,7 +317,7 @@ __device__ __forceinline__ void sep3d_body(const FusedArgs<R>& p, int bx, int by
     for (int c = 0; c < CPT; ++c) {
       R sum = R(0);
 #pragma unroll
-      for (int k = 0; k < W; ++k) sum = __builtin_fma(wv[c + k], tap<R, F>(k), sum);
+      for (int k = 0; k < W; ++k) sum = fmar(wv[c + k], tap<R, F>(k), sum);
       pv[c] = sum;
     }
   };
@@ -352,12 +352,12 @@ __device__ __forceinline__ void sep3d_body(const FusedArgs<R>& p, int bx, int by
           win[c][W - 1 + u] = pv[c];
           R sum = R(0);
 #pragma unroll
-          for (int k = 0; k < W; ++k) sum = __builtin_fma(win[c][u + k], tap<R, F>(k), sum);
+          for (int k = 0; k < W; ++k) sum = fmar(win[c][u + k], tap<R, F>(k), sum);
           if (p.flags & F_A) sum = sum * p.a0;
           if (cout_ok[c]) {
             R* yp = ya + (X0 + tx - p.d0.oy) +
                     (long)p.d0.ny * ((Y0 + ty0 + c - p.d1.oy) + (long)p.d1.ny * (z - p.d2.oy));
-            if (p.flags & F_AY) sum = __builtin_fma(p.a_y, *yp, sum);
+            if (p.flags & F_AY) sum = fmar(p.a_y, *yp, sum);
             *yp = sum;
           }
         }

@@ -38,10 +38,10 @@ __device__ __forceinline__ void wav_step(const V& v, R& o0, R& o1) {
   R t0 = R(0), t1 = R(0);
 #pragma unroll
   for (int s = 0; s < Op::W; ++s) {
-    t0 = __builtin_fma(v(s, 0), C::template c<R>(0, 0, s), t0);
-    t1 = __builtin_fma(v(s, 0), C::template c<R>(1, 0, s), t1);
-    t0 = __builtin_fma(v(s, 1), C::template c<R>(0, 1, s), t0);
-    t1 = __builtin_fma(v(s, 1), C::template c<R>(1, 1, s), t1);
+    t0 = fmar(v(s, 0), C::template c<R>(0, 0, s), t0);
+    t1 = fmar(v(s, 0), C::template c<R>(1, 0, s), t1);
+    t0 = fmar(v(s, 1), C::template c<R>(0, 1, s), t0);
+    t1 = fmar(v(s, 1), C::template c<R>(1, 1, s), t1);
   }
   o0 = t0;
   o1 = t1;
@@ -201,8 +201,8 @@ __device__ __forceinline__ void sepw3d_body(const FusedArgs<R>& p, int bx, int b
             R* y0 = ya + yoff_xy + c1 * ystep1 + zo0;
             R* y1 = ya + yoff_xy + c1 * ystep1 + zo1;
             if (p.flags & F_AY) {
-              o0 = __builtin_fma(p.a_y, *y0, o0);
-              o1 = __builtin_fma(p.a_y, *y1, o1);
+              o0 = fmar(p.a_y, *y0, o0);
+              o1 = fmar(p.a_y, *y1, o1);
             }
             *y0 = o0;
             *y1 = o1;

@@ -45,6 +45,11 @@ LC_DEFINE_FILTER(FCLPG, CLPG)
 LC_DEFINE_FILTER(FCHPG, CHPG)
 #undef LC_DEFINE_FILTER
 
+// fused multiply-add in the storage precision (the float overload keeps
+// FP32 data in FP32 FMAs; energies call the double one explicitly)
+__host__ __device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename R, class F>
 __host__ __device__ __forceinline__ constexpr R tap(int k) {
   if constexpr (sizeof(R) == 8) return F::d[k]; else return F::f[k];
@@ -91,7 +96,7 @@ struct Conv {
   __device__ __forceinline__ static void compute(const R (*win)[1], R* out) {
     R t = R(0);
 #pragma unroll
-    for (int s = 0; s < W; ++s) t = __builtin_fma(win[s][0], tap<R, F>(s), t);
+    for (int s = 0; s < W; ++s) t = fmar(win[s][0], tap<R, F>(s), t);
     out[0] = t;
   }
 };
@@ -105,10 +110,10 @@ struct Dwt {  // input (s_2j, s_2j+1), output (L_i, H_i)
     R lt = R(0), ht = R(0);
 #pragma unroll
     for (int s = 0; s < W; ++s) {
-      lt = __builtin_fma(win[s][0], S::template le<R>(s), lt);
-      ht = __builtin_fma(win[s][0], S::template he<R>(s), ht);
-      lt = __builtin_fma(win[s][1], S::template lo<R>(s), lt);
-      ht = __builtin_fma(win[s][1], S::template ho<R>(s), ht);
+      lt = fmar(win[s][0], S::template le<R>(s), lt);
+      ht = fmar(win[s][0], S::template he<R>(s), ht);
+      lt = fmar(win[s][1], S::template lo<R>(s), lt);
+      ht = fmar(win[s][1], S::template ho<R>(s), ht);
     }
     out[0] = lt;
     out[1] = ht;
@@ -124,10 +129,10 @@ struct Idwt {  // input (L_j, H_j), output (s_2i, s_2i+1)
     R et = R(0), ot = R(0);
 #pragma unroll
     for (int s = 0; s < W; ++s) {
-      et = __builtin_fma(win[s][0], S::template lo<R>(s), et);
-      ot = __builtin_fma(win[s][0], S::template le<R>(s), ot);
-      et = __builtin_fma(win[s][1], S::template ho<R>(s), et);
-      ot = __builtin_fma(win[s][1], S::template he<R>(s), ot);
+      et = fmar(win[s][0], S::template lo<R>(s), et);
+      ot = fmar(win[s][0], S::template le<R>(s), ot);
+      et = fmar(win[s][1], S::template ho<R>(s), et);
+      ot = fmar(win[s][1], S::template he<R>(s), ot);
     }
     out[0] = et;
     out[1] = ot;

@@ -1,0 +1,370 @@
+// lc_fused_impl.hpp -- kernels, launchers and variant dispatch of the fused
+// 3D convolution operators (internal).  Included once per filter by
+// lc_fused_f*.hip, which instantiate it, so the variants compile in parallel;
+// lc_fused.hip holds the filter-independent host code.
+#pragma once
+
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "lc_body_conv.hpp"
+#include "lc_body_sum2.hpp"
+
+namespace lc {
+
+hipError_t launch_batch_dots(const double* part, const int* first, double* const* dots, int count,
+                             hipStream_t st);
+
+// ====================================================================== kernels
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD, int WPE>
+__global__ __launch_bounds__(TX * TY / CPT) __attribute__((amdgpu_waves_per_eu(WPE))) void sum3d_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[sum3d_lds_bytes<R, F, TX, TY, CPT, QX, BZ, HD>()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sum3d_body<R, F, TX, TY, CPT, QX, BZ, HD>(p, bx, by, zc, arr,
+                                           (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc, lds);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD, int WPE>
+__global__ __launch_bounds__(TX * TY / CPT) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[sep3d_lds_bytes<R, F, TX, TY, CPT, QX, BZ, AD>()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sep3d_body<R, F, TX, TY, CPT, QX, BZ, AD>(p, bx, by, zc, arr, lds);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD, int WPE>
+__global__ __launch_bounds__(TX * TY / CPT) __attribute__((amdgpu_waves_per_eu(WPE))) void sum3d_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  const int bx = tile % p.tiles_x, by = tile / p.tiles_x;
+  __shared__ __align__(16) unsigned char lds[sum3d_lds_bytes<R, F, TX, TY, CPT, QX, BZ, HD>()];
+  sum3d_body<R, F, TX, TY, CPT, QX, BZ, HD>(p, bx, by, zc, 0, (long)local, lds);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD, int WPE>
+__global__ __launch_bounds__(TX * TY / CPT) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[sep3d_lds_bytes<R, F, TX, TY, CPT, QX, BZ, AD>()];
+  sep3d_body<R, F, TX, TY, CPT, QX, BZ, AD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+}
+
+// ====================================================================== launch
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int HD, int WPE>
+hipError_t launch_sum3d(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
+                               R a_y, double* dot, hipStream_t st, double* scratch,
+                               size_t scratch_bytes) {
+  FusedArgs<R> p{};
+  p.x = x; p.y = y;
+  p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+  p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+  p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+  p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+  p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  p.narr = narr;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  p.a0 = (R)a[0]; p.a1 = (R)a[1]; p.a2 = (R)a[2];
+  p.a_x = a_x; p.a_y = a_y;
+  p.flags = (a_x != R(0) ? F_AX : 0) | (a_y != R(0) ? F_AY : 0) | (dot ? F_DOT : 0);
+  p.nparts = tiles * zch;
+  if (dot) {
+    if (scratch_bytes < sizeof(double) * 3 * (size_t)p.nparts) return hipErrorInvalidValue;
+    p.part = scratch;
+  }
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  sum3d_fused<R, F, TX, TY, CPT, QX, BZ, HD, WPE><<<grid, TX * TY / CPT, 0, st>>>(p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !dot) return e;
+  return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
+}
+
+template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD, int WPE>
+hipError_t launch_sep3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                               hipStream_t st) {
+  FusedArgs<R> p{};
+  p.x = x; p.y = y;
+  p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+  p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+  p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+  p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+  p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  p.narr = narr;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  p.a0 = a; p.a_y = a_y;
+  p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0);
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  sep3d_fused<R, F, TX, TY, CPT, QX, BZ, AD, WPE><<<grid, TX * TY / CPT, 0, st>>>(p);
+  return hipGetLastError();
+}
+
+// ---- two-plane kinetic form (lc_body_sum2.hpp)
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sum3d2_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC>(p, bx, by, zc, arr,
+                                        (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sum3d2_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC>
+hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
+                         double* dot, hipStream_t st, double* scratch, size_t scratch_bytes) {
+  FusedArgs<R> p{};
+  p.x = x; p.y = y;
+  p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+  p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+  p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+  p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+  p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  p.narr = narr;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  p.a0 = (R)a[0]; p.a1 = (R)a[1]; p.a2 = (R)a[2];
+  p.a_x = a_x; p.a_y = a_y;
+  p.flags = (a_x != R(0) ? F_AX : 0) | (a_y != R(0) ? F_AY : 0) | (dot ? F_DOT : 0);
+  p.nparts = tiles * zch;
+  if (dot) {
+    if (scratch_bytes < sizeof(double) * 3 * (size_t)p.nparts) return hipErrorInvalidValue;
+    p.part = scratch;
+  }
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  sum3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE, KC><<<grid, TX * TY, 0, st>>>(p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !dot) return e;
+  return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
+}
+
+// ====================================================================== variants
+// Compiled variants of the two fused forms: tile TX x TY, CPT rows per lane,
+// QX outputs per x-task, BZ planes per window slide, HD / AD prefetch depth.
+// Every entry of these tables also has a batched form; indices 5..7 are the
+// wave-specialised kernels of lc_fused_ws.hip (single calls only).  The
+// variant of a call comes from the tuner's table for its exact shape, else
+// LIBCONV_{SUM3D,SEP3D}_VARIANT, else a per-form default (lc_fused.hip).
+#define LC_SUM_VARIANTS(X)                                                          \
+  X(0, 32, 16, 2, 2, 4, 2, 1) X(1, 32, 16, 2, 2, 4, 1, 1) X(2, 32, 8, 2, 2, 4, 2, 1)    \
+  X(3, 64, 8, 2, 4, 4, 2, 1) X(4, 32, 16, 2, 2, 2, 2, 1) X(8, 32, 16, 1, 2, 2, 1, 1)    \
+  X(9, 32, 8, 1, 2, 2, 1, 1) X(10, 64, 8, 1, 4, 2, 1, 1) X(11, 32, 16, 1, 4, 4, 1, 1)  \
+  X(12, 32, 16, 2, 2, 4, 1, 2) X(13, 32, 16, 1, 2, 2, 1, 4) X(14, 32, 16, 1, 4, 2, 1, 3) \
+  X(15, 32, 8, 1, 2, 2, 1, 4)
+#define LC_SEP_VARIANTS(X)                                                          \
+  X(0, 32, 16, 2, 4, 4, 4, 1) X(1, 32, 16, 2, 4, 4, 1, 1) X(2, 32, 8, 2, 4, 4, 2, 1)    \
+  X(3, 64, 8, 2, 4, 4, 2, 1) X(4, 32, 16, 2, 4, 4, 2, 1) X(8, 32, 16, 1, 4, 2, 1, 1)    \
+  X(9, 32, 8, 1, 4, 2, 1, 1) X(10, 64, 8, 1, 4, 2, 1, 1) X(11, 32, 16, 1, 4, 4, 1, 1)  \
+  X(12, 32, 16, 2, 4, 4, 1, 2) X(13, 64, 16, 1, 4, 2, 1, 2) X(14, 32, 32, 2, 4, 2, 1, 2) \
+  X(15, 64, 32, 2, 4, 2, 1, 1)
+#define LC_WS_VARIANTS(X) X(5, 32, 8) X(6, 64, 8) X(7, 32, 16)
+// two-plane kinetic form (sum only): tile, QX / QY outputs per task, QZ planes
+// per step, waves per EU
+#define LC_SUM2_VARIANTS(X)                                                             \
+  X(16, 32, 16, 4, 4, 2, 4, 0) X(17, 32, 16, 4, 4, 2, 2, 0) X(18, 64, 8, 4, 4, 2, 4, 0)     \
+  X(19, 32, 16, 8, 8, 4, 2, 0) X(20, 32, 16, 4, 4, 2, 4, 8) X(21, 32, 16, 8, 8, 4, 2, 8)    \
+  X(22, 16, 16, 4, 4, 2, 4, 8) X(23, 16, 16, 4, 4, 2, 2, 0)
+constexpr int kFusedVariantCount = 24;
+
+template <typename R, class F>
+hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
+                                R a_x, R a_y, double* dot, hipStream_t st, double* sc, size_t sb) {
+  switch (v) {
+#define X(I, TX, TY, CPT, QX, BZ, HD, WPE) \
+  case I: return launch_sum3d<R, F, TX, TY, CPT, QX, BZ, HD, WPE>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    LC_SUM_VARIANTS(X)
+#undef X
+#define X(I, TX, TY) \
+  case I: return launch_sum3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
+    LC_WS_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC) \
+  case I: return launch_sum3d2<R, F, TX, TY, QX, QY, QZ, WPE, KC>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    LC_SUM2_VARIANTS(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R, class F>
+hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
+                                hipStream_t st) {
+  switch (v) {
+#define X(I, TX, TY, CPT, QX, BZ, AD, WPE) \
+  case I: return launch_sep3d<R, F, TX, TY, CPT, QX, BZ, AD, WPE>(b, narr, x, y, a, a_y, st);
+    LC_SEP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY) \
+  case I: return launch_sep3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
+    LC_WS_VARIANTS(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R, class F, int TX, int TY>
+hipError_t fused_stage_build(int kind, const BatchItem<R>* it, int count, BatchStage* s) {
+  std::vector<FusedArgs<R>> tab(count);
+  std::vector<int> first(count + 1, 0);
+  std::vector<double*> dots(count, nullptr);
+  long tiles_all = 0;
+  for (int o = 0; o < count; ++o) {
+    const Box3& b = it[o].b;
+    FusedArgs<R>& p = tab[o];
+    p = FusedArgs<R>{};
+    p.x = it[o].x; p.y = it[o].y;
+    p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+    p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+    p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+    p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+    p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+    p.narr = 1;
+    p.tiles_x = (p.d0.nout + TX - 1) / TX;
+    p.tiles_y = (p.d1.nout + TY - 1) / TY;
+    p.a0 = (R)it[o].a[0]; p.a1 = (R)it[o].a[1]; p.a2 = (R)it[o].a[2];
+    p.a_x = (R)it[o].a_x; p.a_y = (R)it[o].a_y;
+    if (kind == BK_SUM)
+      p.flags = (p.a_x != R(0) ? F_AX : 0) | (p.a_y != R(0) ? F_AY : 0) | (it[o].dot ? F_DOT : 0);
+    else
+      p.flags = (p.a0 != R(1) ? F_A : 0) | (p.a_y != R(0) ? F_AY : 0);
+    dots[o] = it[o].dot;
+    if (it[o].dot) s->any_dot = true;
+    tiles_all += (long)p.tiles_x * p.tiles_y;
+  }
+  // the same number of z chunks for every item: enough blocks for ~2 per CU
+  const long chunks = std::max(1L, (2L * num_cu() + tiles_all - 1) / std::max(1L, tiles_all));
+  long total = 0;
+  for (int o = 0; o < count; ++o) {
+    FusedArgs<R>& p = tab[o];
+    const long c = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / (4 * F::N))));
+    p.zchunk = std::max(1, (int)((p.d2.nout + c - 1) / c));
+    const long nb = (long)p.tiles_x * p.tiles_y * ((p.d2.nout + p.zchunk - 1) / p.zchunk);
+    first[o] = (int)total;
+    p.nparts = nb;
+    total += nb;
+  }
+  first[count] = (int)total;
+  if (total > 0x7fffffffL) return hipErrorInvalidValue;
+  s->blocks = (int)total;
+  if (kind == BK_SUM && s->any_dot) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->part), sizeof(double) * 3 * std::max(total, 1L));
+    if (e != hipSuccess) return e;
+    for (int o = 0; o < count; ++o)
+      if (it[o].dot) tab[o].part = s->part + 3L * first[o];
+    e = hipMalloc(reinterpret_cast<void**>(&s->dots), sizeof(double*) * count);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(s->dots, dots.data(), sizeof(double*) * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  return batch_upload(tab.data(), sizeof(FusedArgs<R>) * count, first.data(), count, s);
+}
+
+template <typename R, class F>
+hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count, BatchStage* s,
+                                hipStream_t st) {
+  if (build) {
+    if (kind == BK_SUM) {
+      switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, HD, WPE) \
+  case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
+        LC_SUM_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC) \
+  case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
+        LC_SUM2_VARIANTS(X)
+#undef X
+      }
+    } else {
+      switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, AD, WPE) \
+  case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
+        LC_SEP_VARIANTS(X)
+#undef X
+      }
+    }
+    return hipErrorInvalidValue;
+  }
+  if (s->blocks == 0) return hipSuccess;
+  const FusedArgs<R>* tab = static_cast<const FusedArgs<R>*>(s->tab);
+  if (kind == BK_SUM) {
+    switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, HD, WPE)                                                                   \
+  case I:                                                                                              \
+    sum3d_batch<R, F, TX, TY, CPT, QX, BZ, HD, WPE><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SUM_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC)                                                             \
+  case I:                                                                                            \
+    sum3d2_batch<R, F, TX, TY, QX, QY, QZ, WPE, KC><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SUM2_VARIANTS(X)
+#undef X
+      default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !s->any_dot) return e;
+    return launch_batch_dots(s->part, s->first, s->dots, s->count, st);
+  } else {
+    switch (s->variant) {
+#define X(I, TX, TY, CPT, QX, BZ, AD, WPE)                                                                   \
+  case I:                                                                                              \
+    sep3d_batch<R, F, TX, TY, CPT, QX, BZ, AD, WPE><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SEP_VARIANTS(X)
+#undef X
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+#define LC_FUSED_INSTANTIATE(R, F)                                                                      \
+  template hipError_t sum3d_variant<R, F>(int, const Box3&, int, const R*, R*, const double*, R, R,    \
+                                          double*, hipStream_t, double*, size_t);                      \
+  template hipError_t sep3d_variant<R, F>(int, const Box3&, int, const R*, R*, R, R, hipStream_t);     \
+  template hipError_t fused_stage_t<R, F>(bool, int, const BatchItem<R>*, int, BatchStage*, hipStream_t);
+
+}  // namespace lc

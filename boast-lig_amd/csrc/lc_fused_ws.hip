@@ -71,17 +71,14 @@ __device__ __forceinline__ void ws_coords(int tiles_x, int& bx, int& by, int& zc
   by = tile / tiles_x;
 }
 
-// sum of W taps from a register window, split over two accumulators so the
-// FMA chain is not one long dependency
+// sum of W taps from a register window, one FMA chain in tap order (the
+// order of every other variant, so that all variants give identical bits)
 template <typename R, class F>
 __device__ __forceinline__ R ztaps(const R* w) {
-  R s0 = R(0), s1 = R(0);
+  R s = R(0);
 #pragma unroll
-  for (int k = 0; k < F::N; k += 2) {
-    s0 = __builtin_fma(w[k], tap<R, F>(k), s0);
-    if (k + 1 < F::N) s1 = __builtin_fma(w[k + 1], tap<R, F>(k + 1), s1);
-  }
-  return s0 + s1;
+  for (int k = 0; k < F::N; ++k) s = fmar(w[k], tap<R, F>(k), s);
+  return s;
 }
 
 // ===================================================================== sum3d
@@ -166,16 +163,16 @@ __global__ __launch_bounds__((SumWs<R, F, TX, TY, QX, QY, HD>::NT)) void sum3d_w
       const R t1 = SY[ty * SW + tx] * p.a1;
       const R t2 = zt * p.a2;
       if (want_dot && has_x) {
-        d0 = __builtin_fma((double)xc, (double)t0, d0);
-        d1 = __builtin_fma((double)xc, (double)t1, d1);
-        d2 = __builtin_fma((double)xc, (double)t2, d2);
+        d0 = fmar((double)xc, (double)t0, d0);
+        d1 = fmar((double)xc, (double)t1, d1);
+        d2 = fmar((double)xc, (double)t2, d2);
       }
-      if ((p.flags & F_AX) && has_x) t0 = __builtin_fma(xc, p.a_x, t0);
+      if ((p.flags & F_AX) && has_x) t0 = fmar(xc, p.a_x, t0);
       R tot = t1 + t0;
       tot = t2 + tot;
       if (out_ok) {
         R* yp = yp0 + ys * (z - p.d2.oy);
-        if (p.flags & F_AY) tot = __builtin_fma(p.a_y, *yp, tot);
+        if (p.flags & F_AY) tot = fmar(p.a_y, *yp, tot);
         *yp = tot;
       }
 #pragma unroll
@@ -236,7 +233,7 @@ __global__ __launch_bounds__((SumWs<R, F, TX, TY, QX, QY, HD>::NT)) void sum3d_w
             const R v = row[k];
 #pragma unroll
             for (int q = 0; q < QX; ++q)
-              if (k - q >= 0 && k - q < W) acc[q] = __builtin_fma(v, tap<R, F>(k - q), acc[q]);
+              if (k - q >= 0 && k - q < W) acc[q] = fmar(v, tap<R, F>(k - q), acc[q]);
           }
 #pragma unroll
           for (int q = 0; q < QX; ++q) SX[r * SW + c0 + q] = acc[q];
@@ -254,7 +251,7 @@ __global__ __launch_bounds__((SumWs<R, F, TX, TY, QX, QY, HD>::NT)) void sum3d_w
             const R v = colp[k * TW];
 #pragma unroll
             for (int q = 0; q < QY; ++q)
-              if (k - q >= 0 && k - q < W) acc[q] = __builtin_fma(v, tap<R, F>(k - q), acc[q]);
+              if (k - q >= 0 && k - q < W) acc[q] = fmar(v, tap<R, F>(k - q), acc[q]);
           }
 #pragma unroll
           for (int q = 0; q < QY; ++q) SY[(r0 + q) * SW + c] = acc[q];
@@ -378,7 +375,7 @@ __global__ __launch_bounds__((SepWs<R, F, TX, TY, QX, QY, AD>::NT)) void sep3d_w
             const R v = row[k];
 #pragma unroll
             for (int q = 0; q < QX; ++q)
-              if (k - q >= 0 && k - q < W) acc[q] = __builtin_fma(v, tap<R, F>(k - q), acc[q]);
+              if (k - q >= 0 && k - q < W) acc[q] = fmar(v, tap<R, F>(k - q), acc[q]);
           }
 #pragma unroll
           for (int q = 0; q < QX; ++q) B[r * BW + c0 + q] = acc[q];
@@ -395,7 +392,7 @@ __global__ __launch_bounds__((SepWs<R, F, TX, TY, QX, QY, AD>::NT)) void sep3d_w
           R sum = ztaps<R, F>(win);
           if (p.flags & F_A) sum = sum * p.a0;
           R* yp = yp0 + ys * (z - p.d2.oy);
-          if (p.flags & F_AY) sum = __builtin_fma(p.a_y, *yp, sum);
+          if (p.flags & F_AY) sum = fmar(p.a_y, *yp, sum);
           *yp = sum;
         }
       }
@@ -414,7 +411,7 @@ __global__ __launch_bounds__((SepWs<R, F, TX, TY, QX, QY, AD>::NT)) void sep3d_w
           const R v = colp[k * BW];
 #pragma unroll
           for (int q = 0; q < QY; ++q)
-            if (k - q >= 0 && k - q < W) acc[q] = __builtin_fma(v, tap<R, F>(k - q), acc[q]);
+            if (k - q >= 0 && k - q < W) acc[q] = fmar(v, tap<R, F>(k - q), acc[q]);
         }
 #pragma unroll
         for (int q = 0; q < QY; ++q) P[(r0 + q) * BW + c] = acc[q];

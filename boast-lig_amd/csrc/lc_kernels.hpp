@@ -128,7 +128,7 @@ hipError_t sepw3d(int kind, int id, const Box3& b, int narr, const R* x, R* y, R
 // Forms of the 3D operators the tuner selects kernels for.
 enum Form : int { FORM_SEP = 0, FORM_SUM = 1, FORM_DWT = 2, FORM_IDWT = 3 };
 int variant_count(int form);          // single-call variants
-int batch_variant_count(int form);    // variants 0..k-1 also have batched kernels
+bool has_batch_form(int form, int v);  // the variant also has a batched kernel
 int wav_variant_count();
 // variant a call of this shape uses: tuner table, else environment, else default
 int select_variant(int prec, int form, int id, const Box3& b);

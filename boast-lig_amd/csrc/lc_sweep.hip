@@ -119,14 +119,14 @@ __global__ __launch_bounds__(256) void sweep_strided(SweepArgs<R> p) {
             const bool has_x = BC != GROW || (i >= 0 && i < p.n);
             if (has_x) {
               const R xc = w2[-LO][0];
-              if (want_dot) dot = __builtin_fma((double)xc, (double)o[0], dot);
-              if (p.flags & F_AX) o[0] = __builtin_fma(xc, p.a_x, o[0]);
+              if (want_dot) dot = fmar((double)xc, (double)o[0], dot);
+              if (p.flags & F_AX) o[0] = fmar(xc, p.a_x, o[0]);
             }
           }
 #pragma unroll
           for (int c = 0; c < OUT; ++c) {
             R* yp = yb + out_elem<K>(i - p.oy, c, p.ny) * p.L;
-            if (p.flags & F_AY) o[c] = __builtin_fma(p.a_y, *yp, o[c]);
+            if (p.flags & F_AY) o[c] = fmar(p.a_y, *yp, o[c]);
             *yp = o[c];
           }
         }
@@ -207,14 +207,14 @@ __global__ __launch_bounds__(256) void sweep_contig(SweepArgs<R> p, int tpl_log2
           const bool has_x = BC != GROW || (i >= 0 && i < p.n);
           if (has_x) {
             const R xc = w2[-LO][0];
-            if (want_dot) dot = __builtin_fma((double)xc, (double)o[0], dot);
-            if (p.flags & F_AX) o[0] = __builtin_fma(xc, p.a_x, o[0]);
+            if (want_dot) dot = fmar((double)xc, (double)o[0], dot);
+            if (p.flags & F_AX) o[0] = fmar(xc, p.a_x, o[0]);
           }
         }
 #pragma unroll
         for (int c = 0; c < OUT; ++c) {
           R* yp = yb + out_elem<K>(i - p.oy, c, p.ny);
-          if (p.flags & F_AY) o[c] = __builtin_fma(p.a_y, *yp, o[c]);
+          if (p.flags & F_AY) o[c] = fmar(p.a_y, *yp, o[c]);
           *yp = o[c];
         }
       }

@@ -4,8 +4,9 @@
 // ConvolutionOperator1d#optimize, BigDFT/GenericConvolution2.rb:744-837,
 // :1223-1267): the search space is the set of compiled kernel variants of an
 // operator form (tile shape, outputs per lane, window step, prefetch depth);
-// every variant runs `repeat` times on random data of the caller's shape, the
-// minimum time is kept (as the reference keeps the min of its repeats), and
+// every variant runs on random data of the caller's shape, in three rounds of
+// `repeat` back-to-back launches (steady clocks); the minimum round average
+// is kept (as the reference keeps the min of its repeats), and
 // the fastest variant is recorded for that exact shape.  Later calls of the
 // same shape use it.  The table persists as JSON, one entry per line, and
 // LIBCONV_TUNE_FILE names a file loaded at first use.
@@ -168,15 +169,21 @@ hipError_t tune_md(int form, int id, const Box3& b, int repeat, int* best, doubl
       (void)hipGetLastError();
       continue;  // the variant does not take this shape
     }
+    // steady state: `repeat` back-to-back launches between two events give
+    // one average; the minimum of three such averages is kept
     double tmin = std::numeric_limits<double>::infinity();
-    for (int r = 0; r < std::max(1, repeat) && err == hipSuccess; ++r) {
+    const int reps = std::max(1, repeat);
+    for (int w = 0; w < 2 && err == hipSuccess; ++w)
+      err = launch_variant<R>(form, id, v, b, x, y, dot, sc, sb, st);
+    for (int r = 0; r < 3 && err == hipSuccess; ++r) {
       err = hipEventRecord(e0, st);
-      if (err == hipSuccess) err = launch_variant<R>(form, id, v, b, x, y, dot, sc, sb, st);
+      for (int k = 0; k < reps && err == hipSuccess; ++k)
+        err = launch_variant<R>(form, id, v, b, x, y, dot, sc, sb, st);
       if (err == hipSuccess) err = hipEventRecord(e1, st);
       if (err == hipSuccess) err = hipEventSynchronize(e1);
       float ms = 0.f;
       if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-      if (err == hipSuccess) tmin = std::min(tmin, (double)ms);
+      if (err == hipSuccess) tmin = std::min(tmin, (double)ms / reps);
     }
     (*times)[v] = tmin;
     if (tmin < bt) {

@@ -282,9 +282,10 @@ int lc_batch_stage_variant(const lc_batch *plan, int stage);
  * In the spirit of BOAST's optimizer (GenericOptimization and
  * ConvolutionOperator1d#optimize, BigDFT/GenericConvolution2.rb:744-837,
  * :1223-1267): the compiled kernel variants of a 3D operator form (tile
- * shape, outputs per lane, window step, prefetch depth) each run `repeat`
- * times on random data of the given shape; the minimum time per variant is
- * kept and the fastest variant is recorded for that exact shape, which later
+ * shape, outputs per lane, window step, prefetch depth) each run on random
+ * data of the given shape in three rounds of `repeat` back-to-back launches;
+ * the minimum round average per variant is kept and the fastest variant is
+ * recorded for that exact shape, which later
  * calls of the shape then use.  times[] (may be NULL) receives the min time
  * of each variant (ms; infinity where a variant rejects the shape).  The
  * table is saved and loaded as JSON; LIBCONV_TUNE_FILE names a file loaded

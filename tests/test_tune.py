@@ -17,16 +17,17 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 
 
 def test_variant_counts(lc):
-    assert lc.variant_count(lc.FORM_SEP) == 8
-    assert lc.variant_count(lc.FORM_SUM) == 8
+    assert lc.variant_count(lc.FORM_SEP) == 16
+    assert lc.variant_count(lc.FORM_SUM) == 24
     assert lc.variant_count(lc.FORM_DWT) == 4
     assert lc.variant_count(lc.FORM_IDWT) == 4
 
 
 def test_default_kernel_names(lc):
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_fused<double, lc::FD2, 32, 16, 2, 2, 4, 1>'
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<double, lc::FD2, 32, 16, 4, 4, 2, 2, 0>'
+    assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_ws<float, lc::FD2, 32, 16, 4, 4, 2>'
     assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
-        'sep3d_fused<float, lc::FMD, 32, 16, 2, 4, 4, 1>'
+        'sep3d_fused<float, lc::FMD, 32, 16, 2, 4, 4, 1, 2>'
     assert lc.kernel_name(8, lc.FORM_IDWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).startswith(
         'sepw3d_fused<double, lc::Idwt<')
 
@@ -35,7 +36,7 @@ def test_environment_selects_variant(lc, monkeypatch):
     monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', '6')
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_ws<double, lc::FD2, 64, 8, 4, 4, 2>'
     monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', '99')   # out of range: default
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('32, 16, 2, 2, 4, 1>')
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('32, 16, 4, 4, 2, 2, 0>')
 
 
 def test_json_cache_round_trip(lc, tmp_path, monkeypatch):
@@ -49,7 +50,7 @@ def test_json_cache_round_trip(lc, tmp_path, monkeypatch):
                  ' {"prec": 8, "form": 1, "op": 2, "n": [8, 8, 8], "bc": [0, 0, 0], '
                  '"nx": [8, 8, 8], "ny": [8, 8, 8], "variant": 42, "ms": 0.1}\n]}\n')
     lc.tune_load(str(p))
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_fused<double, lc::FD2, 64, 8, 2, 4, 4, 2>'
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_fused<double, lc::FD2, 64, 8, 2, 4, 4, 2, 1>'
     assert ', 32, 4, 4, 2, 2>' in lc.kernel_name(8, lc.FORM_DWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3)
     # the entry with an unknown variant is ignored; other precisions are separate
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, [8] * 3, 0, [8] * 3, [8] * 3).startswith('sum3d_ws<')
@@ -57,10 +58,10 @@ def test_json_cache_round_trip(lc, tmp_path, monkeypatch):
     q = tmp_path / 'again.json'
     lc.tune_save(str(q))
     text = q.read_text()
-    assert text.count('"variant"') == 2 and '"kernel": "sum3d_fused<double, lc::FD2, 64, 8, 2, 4, 4, 2>"' in text
+    assert text.count('"variant"') == 2 and '"kernel": "sum3d_fused<double, lc::FD2, 64, 8, 2, 4, 4, 2, 1>"' in text
     lc.tune_clear()
     lc.tune_load(str(q))
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('64, 8, 2, 4, 4, 2>')
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('64, 8, 2, 4, 4, 2, 1>')
 
 
 def test_tuner_rejects_bad_arguments(lc, tmp_path):
