@@ -903,7 +903,9 @@ int stage_default_variant(const lc_batch_item* it, int count, int prec) {
   const int form = item_form(it[big]);
   const int v = select_variant(prec, form, it[big].op, item_box(it[big]));
   if (has_batch_form(form, v)) return v;
-  if (form == FORM_SUM) return prec == 4 ? 21 : 17;  // fastest batched kinetic forms
+  bool fit = true;
+  for (int i = 0; i < count; ++i) fit = fit && plane_fits(item_box(it[i]), prec);
+  if (form == FORM_SUM) return !fit ? 8 : prec == 4 ? 22 : 23;  // fastest batched kinetic forms
   return form == FORM_SEP ? 12 : 0;
 }
 

@@ -227,7 +227,7 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
     if (k == BK_SEP && (id == OP_MD || id == OP_IMD)) {
       pi.body = id == OP_MD ? PB_SEP_MD : PB_SEP_IMD;
       tab[i] = id == OP_MD ? conv_args<R, FMD, 32, 16>(it[i], false) : conv_args<R, FIMD, 32, 16>(it[i], false);
-    } else if (k == BK_SUM && id == OP_D2) {
+    } else if (k == BK_SUM && id == OP_D2 && plane_fits(it[i].b, sizeof(R))) {
       pi.body = PB_SUM_D2;
       tab[i] = conv_args<R, FD2, 16, 16>(it[i], true);
       pi.dot = it[i].dot;

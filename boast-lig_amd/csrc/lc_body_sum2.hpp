@@ -31,8 +31,8 @@ struct Sum2Geo {
   static constexpr int TH = TY + HL + HR;            // tile rows
   static constexpr int SW = TX | 1;                  // x-/y-term row stride (odd)
   static constexpr int NHP = TY * (HL + HR) + TX * (HL + HR);  // halo values per plane
-  static constexpr int NH = QZ * NHP;                // per step
-  static constexpr int NHT = (NH + NT - 1) / NT;     // per lane
+  static constexpr int NHL = (NHP + NT - 1) / NT;    // per lane per plane
+  static constexpr int NHT = QZ * NHL;               // per lane per step
   static constexpr int NXT = QZ * TY * (TX / QX);    // x-tasks per step
   static constexpr int NYT = QZ * TX * (TY / QY);    // y-tasks per step
   static constexpr int WS = W - 1 + QZ;              // z window
@@ -51,7 +51,7 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
                                             long bid, unsigned char* lds) {
   using G = Sum2Geo<R, F, TX, TY, QX, QY, QZ>;
   constexpr int W = G::W, HL = G::HL, HR = G::HR, TW = G::TW, TH = G::TH, SW = G::SW;
-  constexpr int NT = G::NT, NHP = G::NHP, NHT = G::NHT, WS = G::WS;
+  constexpr int NT = G::NT, NHP = G::NHP, NHL = G::NHL, NHT = G::NHT, WS = G::WS;
   R* T = reinterpret_cast<R*>(lds);                                   // [QZ][TH][TW]
   R* SX = reinterpret_cast<R*>(lds + lds_align(sizeof(R) * QZ * TH * TW));  // [QZ][TY][SW]
   R* SY = SX + lds_align(sizeof(R) * QZ * TY * SW) / sizeof(R);
@@ -70,48 +70,38 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
   const int tx = t % TX, ty = t / TX;
   const int s0c = fsrc(p.d0, X0 + tx), s1c = fsrc(p.d1, Y0 + ty);
   const int coff = (s0c >= 0 && s1c >= 0) ? s0c + p.d0.nx * s1c : -1;
+  const int cboff = plane_boff<R>(coff);
   const bool out_ok = (X0 + tx < p.d0.olo + p.d0.nout) && (Y0 + ty < p.d1.olo + p.d1.nout);
-  auto col_at = [&](int z) -> R {
-    const int s2 = fsrc(p.d2, z);
-    return (s2 >= 0 && coff >= 0) ? xa[(long)s2 * sxy + coff] : R(0);
-  };
+  auto col_at = [&](int z) -> R { return plane_ld<R>(plane_rsrc(xa, sxy, fsrc(p.d2, z)), cboff); };
 
-  // ---- halo values of this lane: element e of a step (plane q = e / NHP)
-  int hslot[NHT], hoff[NHT], hq[NHT];
+  // ---- halo values of this lane: element t + j NT of every plane of a step
+  // (register k = q NHL + j holds plane q: the plane is known statically)
+  int hslot[NHL], hoff[NHL];
 #pragma unroll
-  for (int k = 0; k < NHT; ++k) {
-    const int e = t + k * NT;
-    const bool valid = e < G::NH;
-    const int q = valid ? e / NHP : 0, f = valid ? e % NHP : 0;
-    int r, c;
+  for (int j = 0; j < NHL; ++j) {
+    const int f = t + j * NT;
+    const bool valid = f < NHP;
+    int r = 0, c = 0;
     if (f < TY * (HL + HR)) {  // x-halo: rows of the tile, columns left / right
       r = f / (HL + HR);
       const int u = f % (HL + HR);
       c = u < HL ? u - HL : TX + (u - HL);
-    } else {                   // y-halo: columns of the tile, rows above / below
+    } else if (valid) {        // y-halo: columns of the tile, rows above / below
       const int g = f - TY * (HL + HR);
       c = g % TX;
       const int u = g / TX;
       r = u < HL ? u - HL : TY + (u - HL);
     }
-    hq[k] = q;
-    hslot[k] = valid ? (q * TH + r + HL) * TW + (c + HL) : -1;
+    hslot[j] = valid ? (r + HL) * TW + (c + HL) : -1;
     const int s0 = fsrc(p.d0, X0 + c), s1 = fsrc(p.d1, Y0 + r);
-    hoff[k] = (valid && s0 >= 0 && s1 >= 0) ? s0 + p.d0.nx * s1 : -1;
+    hoff[j] = plane_boff<R>((valid && s0 >= 0 && s1 >= 0) ? s0 + p.d0.nx * s1 : -1);
   }
-  auto load_halo = [&](int z0, R* dst) {
-    long pb[QZ];  // plane offsets of the step (uniform), -1 where x~ is zero
+  auto load_halo = [&](int z0, R* dst) {  // zero padding read as out-of-range
 #pragma unroll
     for (int q = 0; q < QZ; ++q) {
-      const int s2 = fsrc(p.d2, z0 + q);
-      pb[q] = s2 >= 0 ? (long)s2 * sxy : -1;
-    }
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(xa, sxy, fsrc(p.d2, z0 + q));
 #pragma unroll
-    for (int k = 0; k < NHT; ++k) {
-      long b = pb[0];
-#pragma unroll
-      for (int q = 1; q < QZ; ++q) b = hq[k] == q ? pb[q] : b;
-      dst[k] = (b >= 0 && hoff[k] >= 0) ? xa[b + hoff[k]] : R(0);
+      for (int j = 0; j < NHL; ++j) dst[q * NHL + j] = plane_ld<R>(rs, hoff[j]);
     }
   };
 
@@ -136,8 +126,10 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
 #pragma unroll
     for (int q = 0; q < QZ; ++q) T[(q * TH + ty + HL) * TW + tx + HL] = win[HL + q];
 #pragma unroll
-    for (int k = 0; k < NHT; ++k)
-      if (hslot[k] >= 0) T[hslot[k]] = hreg[k];
+    for (int q = 0; q < QZ; ++q)
+#pragma unroll
+      for (int j = 0; j < NHL; ++j)
+        if (hslot[j] >= 0) T[q * TH * TW + hslot[j]] = hreg[q * NHL + j];
     // prefetch the next step (planes beyond the chunk are never used)
     if (z0 + QZ < ze) {
 #pragma unroll

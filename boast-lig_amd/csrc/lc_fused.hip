@@ -37,9 +37,10 @@ size_t fused_sum3d_parts(const Box3& b, int narr, int lo, int up, int w) {
   FDim g0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], lo, up);
   FDim g1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], lo, up);
   FDim g2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], lo, up);
-  // upper bound over the variants: smallest tile, most z chunks
-  const long tiles = (long)((g0.nout + 31) / 32) * ((g1.nout + 7) / 8);
-  const long maxc = std::max(1, g2.nout / (4 * w)) + 1;
+  // upper bound over the variants: tiles no smaller than 16 x 8, z chunks no
+  // shorter than 16 planes (two-plane kinetic form) or 4 w (the others)
+  const long tiles = (long)((g0.nout + 15) / 16) * ((g1.nout + 7) / 8);
+  const long maxc = std::max(std::max(1, g2.nout / (4 * w)), g2.nout / 16) + 2;
   (void)narr;
   return (size_t)(tiles * maxc);
 }
@@ -71,11 +72,11 @@ LC_FUSED_EXTERN(float, FD1)
 // variants 0..15 of the fused tables (5..7: wave-specialised)
 int variant_count(int form) {
   if (form == FORM_SUM) return kFusedVariantCount;
-  if (form == FORM_SEP) return 16;
+  if (form == FORM_SEP) return kSepVariantCount;
   return wav_variant_count();
 }
 bool has_batch_form(int form, int v) {
-  if (form == FORM_SEP) return v >= 0 && v < 16 && (v < 5 || v > 7);
+  if (form == FORM_SEP) return v >= 0 && v < kSepVariantCount && (v < 5 || v > 7);
   if (form == FORM_SUM) return v >= 0 && v < kFusedVariantCount && (v < 5 || v > 7);
   return v >= 0 && v < wav_variant_count();
 }
@@ -103,7 +104,7 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
     break;
       LC_SUM_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE, KC)                                                            \
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC)                                                        \
   case I:                                                                                           \
     snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d, %d>",                          \
              batch ? "sum3d2_batch" : "sum3d2_fused", r, filter_tag(id), TX, TY, QX, QY, QZ, WPE, KC); \
@@ -119,6 +120,13 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
              batch ? "sep3d_batch" : "sep3d_fused", r, filter_tag(id), TX, TY, CPT, QX, BZ, AD, WPE); \
     break;
       LC_SEP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE)                                                                \
+  case I:                                                                                           \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>",                              \
+             batch ? "sep3d2_batch" : "sep3d2_fused", r, filter_tag(id), TX, TY, QX, QY, QZ, WPE);   \
+    break;
+      LC_SEP2_VARIANTS(X)
 #undef X
     }
   }
@@ -144,9 +152,12 @@ int select_variant(int prec, int form, int id, const Box3& b) {
   if (form == FORM_SUM || form == FORM_SEP) {
     v = env_variant(form == FORM_SUM ? "LIBCONV_SUM3D_VARIANT" : "LIBCONV_SEP3D_VARIANT", variant_count(form));
     if (v >= 0) return v;
-    // defaults: the fastest variants at 256^3 on MI355X (tuner sweep)
-    if (form == FORM_SEP) return 12;
-    return prec == 4 ? 7 : 17;
+    // defaults: the fastest variants at 256^3 on MI355X (tuner sweep); the
+    // two-plane forms (16 and up) take planes of at most kPlaneMax bytes
+    const bool fit = plane_fits(b, prec);
+    if (form == FORM_SEP) return fit ? 17 : 12;
+    if (!fit) return prec == 4 ? 7 : 8;
+    return prec == 4 ? 22 : 23;
   }
   v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
   return v >= 0 ? v : 0;

@@ -25,6 +25,37 @@ __device__ __forceinline__ int fsrc(const FDim& g, int j) {
   return (s >= 0 && s < g.nx) ? s : -1;  // beyond the stored input: never used
 }
 
+// One input plane as a raw buffer resource (stride 0, 32-bit records).  A
+// buffer load whose byte offset is >= the record count reads zero, so a tile
+// value outside the input (zero padding of the grow / shrink conditions) is
+// loaded without a branch or a select: its offset is kOffNone, or the whole
+// plane is absent (record count 0).  The two-plane bodies use this for their
+// prefetch, where a branch or a select on the loaded value would make the
+// wave wait for the load at once.  Plane sizes are limited to kPlaneMax bytes.
+constexpr int kOffNone = int(0x80000000u);
+constexpr int kBufDword3 = 0x00020000;  // raw 32-bit data format of gfx9 buffers
+
+template <typename R>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const R* xa, long sxy, int s2) {
+  const R* pl = xa + (s2 >= 0 ? (long)s2 * sxy : 0);
+  const int nrec = s2 >= 0 ? int(sxy * (long)sizeof(R)) : 0;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(pl), (short)0, nrec, kBufDword3);
+}
+
+// element at byte offset `boff` of the plane; kOffNone reads zero
+template <typename R>
+__device__ __forceinline__ R plane_ld(__amdgpu_buffer_rsrc_t rs, int boff) {
+  if constexpr (sizeof(R) == 8)
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, boff, 0, 0));
+  else
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, 0, 0));
+}
+
+template <typename R>
+__host__ __device__ __forceinline__ int plane_boff(int off) {  // element offset (< 0: none)
+  return off >= 0 ? off * (int)sizeof(R) : kOffNone;
+}
+
 template <typename R>
 struct FusedArgs {
   const R* x;

@@ -13,6 +13,7 @@
 
 #include "lc_body_conv.hpp"
 #include "lc_body_sum2.hpp"
+#include "lc_body_sep2.hpp"
 
 namespace lc {
 
@@ -148,9 +149,10 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
 }
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC>
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC, int ZC>
 hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
                          double* dot, hipStream_t st, double* scratch, size_t scratch_bytes) {
+  if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p{};
   p.x = x; p.y = y;
   p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
@@ -162,7 +164,17 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
   p.tiles_x = (p.d0.nout + TX - 1) / TX;
   p.tiles_y = (p.d1.nout + TY - 1) / TY;
   const long tiles = (long)p.tiles_x * p.tiles_y;
-  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  if (ZC == 0) {
+    p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  } else {
+    // a z chunk only costs the window prefill (W - 1 column loads per lane):
+    // chunks down to ZC planes put ~8 blocks on every CU
+    const long want = 8L * num_cu();
+    long chunks = (want + tiles * std::max(narr, 1) - 1) / (tiles * std::max(narr, 1));
+    chunks = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / ZC)));
+    p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
+    p.zchunk = (p.zchunk + QZ - 1) / QZ * QZ;
+  }
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   p.a0 = (R)a[0]; p.a1 = (R)a[1]; p.a2 = (R)a[2];
   p.a_x = a_x; p.a_y = a_y;
@@ -178,6 +190,53 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !dot) return e;
   return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
+}
+
+// ---- two-plane separable form (lc_body_sep2.hpp)
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d2_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sep3d2_body<R, F, TX, TY, QX, QY, QZ>(p, bx, by, zc, arr, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d2_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  sep3d2_body<R, F, TX, TY, QX, QY, QZ>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE>
+hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
+  if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
+  FusedArgs<R> p{};
+  p.x = x; p.y = y;
+  p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+  p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+  p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+  p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+  p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  p.narr = narr;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  p.a0 = a; p.a_y = a_y;
+  p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0);
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  sep3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE><<<grid, TX * TY, 0, st>>>(p);
+  return hipGetLastError();
 }
 
 // ====================================================================== variants
@@ -202,11 +261,18 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
 #define LC_WS_VARIANTS(X) X(5, 32, 8) X(6, 64, 8) X(7, 32, 16)
 // two-plane kinetic form (sum only): tile, QX / QY outputs per task, QZ planes
 // per step, waves per EU
-#define LC_SUM2_VARIANTS(X)                                                             \
-  X(16, 32, 16, 4, 4, 2, 4, 0) X(17, 32, 16, 4, 4, 2, 2, 0) X(18, 64, 8, 4, 4, 2, 4, 0)     \
-  X(19, 32, 16, 8, 8, 4, 2, 0) X(20, 32, 16, 4, 4, 2, 4, 8) X(21, 32, 16, 8, 8, 4, 2, 8)    \
-  X(22, 16, 16, 4, 4, 2, 4, 8) X(23, 16, 16, 4, 4, 2, 2, 0)
-constexpr int kFusedVariantCount = 24;
+#define LC_SUM2_VARIANTS(X)                                                                      \
+  X(16, 32, 16, 4, 4, 2, 4, 0, 0) X(17, 32, 16, 4, 4, 2, 2, 0, 0) X(18, 64, 8, 4, 4, 2, 4, 0, 0)       \
+  X(19, 32, 16, 8, 8, 4, 2, 0, 0) X(20, 32, 16, 4, 4, 2, 4, 8, 16) X(21, 32, 16, 8, 8, 4, 2, 8, 32)    \
+  X(22, 16, 16, 4, 4, 2, 4, 8, 16) X(23, 16, 16, 4, 4, 2, 2, 0, 16) X(24, 32, 8, 4, 4, 2, 3, 0, 16)    \
+  X(25, 32, 8, 4, 4, 2, 2, 0, 16)
+// two-plane separable form (sep only): tile, QX / QY outputs per task, QZ
+// planes per step, waves per EU
+#define LC_SEP2_VARIANTS(X)                                                           \
+  X(16, 32, 16, 4, 2, 2, 2) X(17, 32, 16, 4, 4, 2, 2) X(18, 32, 16, 4, 2, 2, 4)          \
+  X(19, 32, 8, 8, 2, 2, 3) X(20, 64, 8, 8, 2, 2, 2) X(21, 32, 32, 4, 4, 2, 2)
+constexpr int kFusedVariantCount = 26;
+constexpr int kSepVariantCount = 22;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
@@ -220,8 +286,8 @@ hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const
   case I: return launch_sum3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb, 4, F::N);
     LC_WS_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE, KC) \
-  case I: return launch_sum3d2<R, F, TX, TY, QX, QY, QZ, WPE, KC>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC) \
+  case I: return launch_sum3d2<R, F, TX, TY, QX, QY, QZ, WPE, KC, ZC>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
     LC_SUM2_VARIANTS(X)
 #undef X
   }
@@ -240,8 +306,19 @@ hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, 
   case I: return launch_sep3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
     LC_WS_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE) \
+  case I: return launch_sep3d2<R, F, TX, TY, QX, QY, QZ, WPE>(b, narr, x, y, a, a_y, st);
+    LC_SEP2_VARIANTS(X)
+#undef X
   }
   return hipErrorInvalidValue;
+}
+
+template <typename R>
+bool items_fit(const BatchItem<R>* it, int count) {  // planes within the buffer-load limit
+  for (int o = 0; o < count; ++o)
+    if (!plane_fits(it[o].b, sizeof(R))) return false;
+  return true;
 }
 
 template <typename R, class F, int TX, int TY>
@@ -311,8 +388,8 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
   case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
         LC_SUM_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE, KC) \
-  case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC) \
+  case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
         LC_SUM2_VARIANTS(X)
 #undef X
       }
@@ -321,6 +398,10 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
 #define X(I, TX, TY, CPT, QX, BZ, AD, WPE) \
   case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
         LC_SEP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE) \
+  case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
+        LC_SEP2_VARIANTS(X)
 #undef X
       }
     }
@@ -336,7 +417,7 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     break;
       LC_SUM_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE, KC)                                                             \
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC)                                                         \
   case I:                                                                                            \
     sum3d2_batch<R, F, TX, TY, QX, QY, QZ, WPE, KC><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
@@ -354,6 +435,12 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     sep3d_batch<R, F, TX, TY, CPT, QX, BZ, AD, WPE><<<s->blocks, TX * TY / CPT, 0, st>>>(tab, s->first, s->count); \
     break;
       LC_SEP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE)                                                                 \
+  case I:                                                                                            \
+    sep3d2_batch<R, F, TX, TY, QX, QY, QZ, WPE><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SEP2_VARIANTS(X)
 #undef X
       default: return hipErrorInvalidValue;
     }

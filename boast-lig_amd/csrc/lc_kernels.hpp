@@ -54,6 +54,11 @@ struct Box3 {
   int n[3], bc[3], nx[3], ny[3];
 };
 
+// the two-plane bodies read input planes through 32-bit buffer records:
+// planes of at most kPlaneMax bytes
+constexpr long kPlaneMax = 0x7fffffffL;
+inline bool plane_fits(const Box3& b, int prec) { return (long)b.nx[0] * b.nx[1] * prec <= kPlaneMax; }
+
 // separable product y = a * F2 F1 F0 x (+ a_y y), fused single pass
 template <typename R>
 hipError_t sep3d(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,

@@ -25,7 +25,7 @@ def lc(gpu):
     libconv.tune_clear()
 
 
-@pytest.mark.parametrize('v', range(24))
+@pytest.mark.parametrize('v', range(26))
 def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v):
     torch = gpu
     rng = np.random.default_rng(1200 + v)
@@ -39,7 +39,7 @@ def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v):
     O.s0s0_md(O.MD, n, bc, nx, ny, 1, x, ref)
     monkeypatch.setenv('LIBCONV_SEP3D_VARIANT', str(v))
     monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', str(v))
-    if v < 16:  # 16.. are kinetic-form kernels only
+    if v < lc.variant_count(lc.FORM_SEP):
         y = torch.zeros(ref.size, dtype=torch.float64, device='cuda')
         lc.conv_md(lc.FIL_MD, n, bc, nx, ny, 1, torch.from_numpy(x).cuda(), y)
         torch.cuda.synchronize()
@@ -79,6 +79,26 @@ def test_every_wavelet_variant_matches_oracle(lc, gpu, monkeypatch, v, inverse):
     assert rel(y.cpu().numpy(), ref) < TOL64
 
 
+def test_all_separable_variants_give_identical_bits(lc, gpu, monkeypatch):
+    torch = gpu
+    rng = np.random.default_rng(1260)
+    n = [41, 30, 35]
+    bc = [1, 0, -1]
+    nx = [n[0], n[1], n[2] + 15]
+    ny = [n[0] + 15, n[1], n[2]]
+    for dt in (torch.float64, torch.float32):
+        x = torch.from_numpy(rng.uniform(-1, 1, int(np.prod(nx)))).to('cuda', dt)
+        outs = []
+        for v in range(lc.variant_count(lc.FORM_SEP)):
+            monkeypatch.setenv('LIBCONV_SEP3D_VARIANT', str(v))
+            y = torch.zeros(int(np.prod(ny)), dtype=dt, device='cuda')
+            lc.conv_md(lc.FIL_MD, n, bc, nx, ny, 1, x, y, a=0.5)
+            outs.append(y)
+        torch.cuda.synchronize()
+        for v, y in enumerate(outs):
+            assert torch.equal(y, outs[0]), (str(dt), v)
+
+
 def test_all_variants_give_identical_bits(lc, gpu, monkeypatch):
     """Every variant sums in the same order, so tuning never changes results."""
     torch = gpu
@@ -103,7 +123,7 @@ def test_all_variants_give_identical_bits(lc, gpu, monkeypatch):
 def test_tune_md_records_fastest(lc, gpu):
     shape = dict(n=[64, 48, 40], bc=0, nx=[64, 48, 40], ny=[64, 48, 40])
     r = lc.tune_md(lc.FORM_SUM, lc.FIL_D2, repeat=2, **shape)
-    assert 0 <= r['best'] < 24 and np.isfinite(r['times'][:16]).all()
+    assert 0 <= r['best'] < 26 and np.isfinite(r['times'][:16]).all()
     assert r['best_ms'] == min(r['times'])
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **shape) != ''
     w = lc.tune_md(lc.FORM_IDWT, lc.WAV_SYM8, repeat=2, prec=4, **shape)
