@@ -14,6 +14,8 @@ input once and writes its output once).  Several buffer sets are rotated so
 that the working set of consecutive steps exceeds the 256 MB Infinity Cache.
 """
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -68,6 +70,19 @@ def chain_args(w):
     return mf, kin
 
 
+@contextlib.contextmanager
+def quiet_host():
+    """Timed regions run without the cyclic garbage collector: a full
+    collection of the interpreter's heap (tens of ms) inside a region stalls
+    the launches and leaves the GPU idle."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
+
+
 def run_gpu(torch, lc, w, dtype, steps, warmup):
     mf, kin = chain_args(w)
     a = [-0.5 / H ** 2] * 3
@@ -102,17 +117,19 @@ def run_gpu(torch, lc, w, dtype, steps, warmup):
     # whole-step timing (no per-op events inside)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    e0.record(st)
-    for i in range(steps):
-        step(i)
-    e1.record(st)
-    torch.cuda.synchronize()
+    with quiet_host():
+        e0.record(st)
+        for i in range(steps):
+            step(i)
+        e1.record(st)
+        torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     # per-operator timing (events between the operators, same stream)
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    for i in range(steps):
-        step(i, evs[i])
-    torch.cuda.synchronize()
+    with quiet_host():
+        for i in range(steps):
+            step(i, evs[i])
+        torch.cuda.synchronize()
     t_mf = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
     t_k = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
     x, y1, y2, dot = sets[0]
@@ -148,9 +165,10 @@ def run_dwt(torch, lc, dtype, steps, warmup, n=96):
         step()
     torch.cuda.synchronize()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    for i in range(steps):
-        step(evs[i])
-    torch.cuda.synchronize()
+    with quiet_host():
+        for i in range(steps):
+            step(evs[i])
+        torch.cuda.synchronize()
     t_f = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
     t_i = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
     err = ((back.double() - s.double()).abs().max() / s.double().abs().max()).item()
@@ -228,13 +246,15 @@ def run_batch(torch, lc, dtype, reps, count=512):
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(st)
-        for _ in range(reps):
-            fn()
-        e1.record(st)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps, (time.perf_counter() - t0) * 1e3 / reps
+        with quiet_host():
+            t0 = time.perf_counter()
+            e0.record(st)
+            for _ in range(reps):
+                fn()
+            e1.record(st)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        return e0.elapsed_time(e1) / reps, (t1 - t0) * 1e3 / reps
 
     t_single, w_single = timed(singles)
     ref = [it['y'].clone() for it in items[::97]]
@@ -309,8 +329,8 @@ def cpu_baseline(O, np, w, x):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--workload', default='c4', choices=['c4', 'c2p', 'c2g'])
     ap.add_argument('--no-oracle', action='store_true', help='skip the FP64 oracle error check')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')

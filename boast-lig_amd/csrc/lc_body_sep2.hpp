@@ -11,8 +11,9 @@
 //   * every lane owns one output column and keeps the last W - 1 filtered
 //     planes in registers; with the step's QZ planes appended it emits the
 //     outputs whose W planes are complete.
-// Three barriers per QZ planes.  Summation orders equal sep3d_body's, so
-// both give identical bits.
+// Three barriers per QZ planes.  The input tiles are loaded PD steps ahead
+// (PD register buffers).  Summation orders equal sep3d_body's, so both give
+// identical bits.
 #pragma once
 
 #include "lc_body_conv.hpp"
@@ -37,7 +38,7 @@ struct Sep2Geo {
   static_assert(TX % QX == 0 && TY % QY == 0 && NT % 64 == 0, "tile shape");
 };
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ>
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int PD>
 __device__ __forceinline__ void sep3d2_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
                                             unsigned char* lds) {
   using G = Sep2Geo<R, F, TX, TY, QX, QY, QZ>;
@@ -83,19 +84,19 @@ __device__ __forceinline__ void sep3d2_body(const FusedArgs<R>& p, int bx, int b
     }
   };
 
-  R areg[NAT];
-  load_step(0, areg);
   R win[W - 1 + QZ];  // the last W - 1 filtered planes, then the step's QZ
 #pragma unroll
   for (int s = 0; s < W - 1 + QZ; ++s) win[s] = R(0);
 
-  for (int s0 = 0; s0 < np; s0 += QZ) {
+  // one step: planes s0 .. s0 + QZ - 1 from the registers `buf`, which then
+  // receive the step PD steps ahead
+  auto step = [&](int s0, R* buf) {
 #pragma unroll
     for (int q = 0; q < QZ; ++q)
 #pragma unroll
       for (int j = 0; j < NAP; ++j)
-        if (aslot[j] >= 0) A[q * TH * TWP + aslot[j]] = areg[q * NAP + j];
-    if (s0 + QZ < np) load_step(s0 + QZ, areg);
+        if (aslot[j] >= 0) A[q * TH * TWP + aslot[j]] = buf[q * NAP + j];
+    if (s0 + PD * QZ < np) load_step(s0 + PD * QZ, buf);
     __syncthreads();
     // ---- x-tasks: QX outputs of a row (halo rows included) of one plane
     for (int task = t; task < G::NXT; task += NT) {
@@ -156,6 +157,15 @@ __device__ __forceinline__ void sep3d2_body(const FusedArgs<R>& p, int bx, int b
     }
 #pragma unroll
     for (int s = 0; s < W - 1; ++s) win[s] = win[s + QZ];
+  };
+
+  R areg[PD][NAT];  // prefetch depth PD: buffer u serves steps u, u + PD, ...
+#pragma unroll
+  for (int u = 0; u < PD; ++u) load_step(u * QZ, areg[u]);
+  for (int s0 = 0; s0 < np; s0 += PD * QZ) {
+#pragma unroll
+    for (int u = 0; u < PD; ++u)
+      if (s0 + u * QZ < np) step(s0 + u * QZ, areg[u]);
   }
 }
 

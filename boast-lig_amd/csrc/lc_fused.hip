@@ -121,10 +121,10 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
     break;
       LC_SEP_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE)                                                                \
-  case I:                                                                                           \
-    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>",                              \
-             batch ? "sep3d2_batch" : "sep3d2_fused", r, filter_tag(id), TX, TY, QX, QY, QZ, WPE);   \
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PD)                                                             \
+  case I:                                                                                                \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d, %d>",                               \
+             batch ? "sep3d2_batch" : "sep3d2_fused", r, filter_tag(id), TX, TY, QX, QY, QZ, WPE, PD);   \
     break;
       LC_SEP2_VARIANTS(X)
 #undef X
@@ -155,9 +155,9 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     // defaults: the fastest variants at 256^3 on MI355X (tuner sweep); the
     // two-plane forms (16 and up) take planes of at most kPlaneMax bytes
     const bool fit = plane_fits(b, prec);
-    if (form == FORM_SEP) return fit ? 17 : 12;
+    if (form == FORM_SEP) return !fit ? 12 : prec == 4 ? 25 : 17;
     if (!fit) return prec == 4 ? 7 : 8;
-    return prec == 4 ? 22 : 23;
+    return prec == 4 ? 21 : 23;
   }
   v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
   return v >= 0 ? v : 0;

@@ -193,16 +193,16 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
 }
 
 // ---- two-plane separable form (lc_body_sep2.hpp)
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE>
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int PD>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d2_fused(
     FusedArgs<R> p) {
   __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
   int bx, by, zc, arr;
   block_coords(p.tiles_x, bx, by, zc, arr);
-  sep3d2_body<R, F, TX, TY, QX, QY, QZ>(p, bx, by, zc, arr, lds);
+  sep3d2_body<R, F, TX, TY, QX, QY, QZ, PD>(p, bx, by, zc, arr, lds);
 }
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE>
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int PD>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d2_batch(
     const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
   const int lin = xcd_linear();
@@ -212,10 +212,10 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = local % tiles, zc = local / tiles;
   __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
-  sep3d2_body<R, F, TX, TY, QX, QY, QZ>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+  sep3d2_body<R, F, TX, TY, QX, QY, QZ, PD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
 }
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE>
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int ZC, int PD>
 hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
   if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p{};
@@ -229,13 +229,21 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
   p.tiles_x = (p.d0.nout + TX - 1) / TX;
   p.tiles_y = (p.d1.nout + TY - 1) / TY;
   const long tiles = (long)p.tiles_x * p.tiles_y;
-  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  if (ZC == 0) {
+    p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  } else {
+    // a chunk recomputes W - 1 filtered planes: shorter chunks buy occupancy
+    const long want = 4L * num_cu();
+    long chunks = (want + tiles * std::max(narr, 1) - 1) / (tiles * std::max(narr, 1));
+    chunks = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / ZC)));
+    p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
+  }
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   p.a0 = a; p.a_y = a_y;
   p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0);
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
-  sep3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE><<<grid, TX * TY, 0, st>>>(p);
+  sep3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE, PD><<<grid, TX * TY, 0, st>>>(p);
   return hipGetLastError();
 }
 
@@ -267,12 +275,15 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
   X(22, 16, 16, 4, 4, 2, 4, 8, 16) X(23, 16, 16, 4, 4, 2, 2, 0, 16) X(24, 32, 8, 4, 4, 2, 3, 0, 16)    \
   X(25, 32, 8, 4, 4, 2, 2, 0, 16)
 // two-plane separable form (sep only): tile, QX / QY outputs per task, QZ
-// planes per step, waves per EU
-#define LC_SEP2_VARIANTS(X)                                                           \
-  X(16, 32, 16, 4, 2, 2, 2) X(17, 32, 16, 4, 4, 2, 2) X(18, 32, 16, 4, 2, 2, 4)          \
-  X(19, 32, 8, 8, 2, 2, 3) X(20, 64, 8, 8, 2, 2, 2) X(21, 32, 32, 4, 4, 2, 2)
+// planes per step, waves per EU, shortest z chunk (0: 4 W, ~2 blocks per CU;
+// else chunks down to ZC planes for ~4 blocks per CU), prefetch depth in steps
+#define LC_SEP2_VARIANTS(X)                                                                        \
+  X(16, 32, 16, 4, 2, 2, 2, 0, 1) X(17, 32, 16, 4, 4, 2, 2, 0, 1) X(18, 32, 16, 4, 2, 2, 4, 0, 1)        \
+  X(19, 32, 8, 8, 2, 2, 3, 0, 1) X(20, 64, 8, 8, 2, 2, 2, 0, 1) X(21, 32, 32, 4, 4, 2, 2, 0, 1)          \
+  X(22, 32, 16, 4, 4, 2, 2, 32, 1) X(23, 32, 16, 4, 4, 2, 3, 32, 1) X(24, 32, 16, 4, 4, 2, 2, 48, 1)     \
+  X(25, 32, 16, 4, 4, 2, 2, 0, 2) X(26, 32, 16, 4, 2, 2, 4, 0, 2) X(27, 32, 16, 4, 4, 2, 2, 48, 2)
 constexpr int kFusedVariantCount = 26;
-constexpr int kSepVariantCount = 22;
+constexpr int kSepVariantCount = 28;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
@@ -306,8 +317,8 @@ hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, 
   case I: return launch_sep3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
     LC_WS_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE) \
-  case I: return launch_sep3d2<R, F, TX, TY, QX, QY, QZ, WPE>(b, narr, x, y, a, a_y, st);
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PD) \
+  case I: return launch_sep3d2<R, F, TX, TY, QX, QY, QZ, WPE, ZC, PD>(b, narr, x, y, a, a_y, st);
     LC_SEP2_VARIANTS(X)
 #undef X
   }
@@ -399,7 +410,7 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
   case I: return fused_stage_build<R, F, TX, TY>(kind, it, count, s);
         LC_SEP_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE) \
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PD) \
   case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
         LC_SEP2_VARIANTS(X)
 #undef X
@@ -436,9 +447,9 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     break;
       LC_SEP_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, QY, QZ, WPE)                                                                 \
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PD)                                                                 \
   case I:                                                                                            \
-    sep3d2_batch<R, F, TX, TY, QX, QY, QZ, WPE><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
+    sep3d2_batch<R, F, TX, TY, QX, QY, QZ, WPE, PD><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
       LC_SEP2_VARIANTS(X)
 #undef X

@@ -226,24 +226,45 @@ __global__ __launch_bounds__(256) void sweep_contig(SweepArgs<R> p, int tpl_log2
   }
 }
 
-// Sum of nparts partials in a fixed order (one block): dot_out[0] = sum.
-__global__ __launch_bounds__(256) void sum_partials(const double* __restrict__ part, long nparts,
-                                                     double* __restrict__ out, int nsets, long stride) {
-  __shared__ double red[4];
-  for (int k = 0; k < nsets; ++k) {
-    double v = 0.0;
-    for (long i = threadIdx.x; i < nparts; i += blockDim.x) v += part[k * stride + i];
+// Sums of nsets (<= 4) sets of nparts partials, in a fixed order (one block):
+// out[k] = sum of set k.  Every thread keeps one accumulator per set and
+// loads four strided partials of each set before adding them (in index
+// order), so the loads of a thread are in flight together.
+constexpr int kPartThreads = 1024;
+__global__ __launch_bounds__(kPartThreads) void sum_partials(const double* __restrict__ part, long nparts,
+                                                              double* __restrict__ out, int nsets, long stride) {
+  __shared__ double red[kPartThreads / 64][4];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  constexpr int U = 4;
+  long i = threadIdx.x;
+  for (; i + (U - 1) * kPartThreads < nparts; i += U * kPartThreads) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-      out[k] = s;
+    for (int k = 0; k < 4; ++k) {
+      if (k < nsets) {
+        double a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = part[k * stride + i + u * kPartThreads];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[k] += a[u];
+      }
     }
-    __syncthreads();
+  }
+  for (; i < nparts; i += kPartThreads)
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < nsets) v[k] += part[k * stride + i];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    if (lane == 0) red[wid][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < nsets) {
+    double s = 0.0;
+    for (int w = 0; w < kPartThreads / 64; ++w) s += red[w][threadIdx.x];
+    out[threadIdx.x] = s;
   }
 }
 
@@ -284,7 +305,7 @@ static hipError_t run_sweep(const SweepArgs<R>& a0, hipStream_t st, double* dot_
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (p.dot_part && dot_out) {
-    sum_partials<<<1, 256, 0, st>>>(p.dot_part, (long)grid.x * grid.y, dot_out, 1, 0);
+    sum_partials<<<1, kPartThreads, 0, st>>>(p.dot_part, (long)grid.x * grid.y, dot_out, 1, 0);
     e = hipGetLastError();
   }
   return e;
@@ -341,7 +362,8 @@ template long sweep_partials_needed<float>(const SweepArgs<float>&, int);
 
 hipError_t sum_partial_sets(const double* part, long nparts, double* out, int nsets, long stride,
                             hipStream_t st) {
-  sum_partials<<<1, 256, 0, st>>>(part, nparts, out, nsets, stride);
+  if (nsets < 1 || nsets > 4) return hipErrorInvalidValue;
+  sum_partials<<<1, kPartThreads, 0, st>>>(part, nparts, out, nsets, stride);
   return hipGetLastError();
 }
 

@@ -7,6 +7,9 @@
 // every variant runs on random data of the caller's shape, in three rounds of
 // `repeat` back-to-back launches (steady clocks); the minimum round average
 // is kept (as the reference keeps the min of its repeats), and
+// the launches rotate over enough input / output sets to exceed the 256 MB
+// last-level cache (MALL) of the chip, so that a variant is timed with its
+// operands in HBM, as in an application, and
 // the fastest variant is recorded for that exact shape.  Later calls of the
 // same shape use it.  The table persists as JSON, one entry per line, and
 // LIBCONV_TUNE_FILE names a file loaded at first use.
@@ -15,6 +18,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <limits>
 #include <map>
 #include <mutex>
@@ -33,6 +37,9 @@ struct Entry {
   int variant;
   double ms;
 };
+
+constexpr long kMallBytes = 256L << 20;  // last-level cache of the chip
+constexpr int kTuneSetsMax = 64;
 
 std::mutex g_mu;
 std::map<std::string, Entry> g_table;
@@ -147,6 +154,9 @@ hipError_t tune_md(int form, int id, const Box3& b, int repeat, int* best, doubl
   const bool wav = form == FORM_DWT || form == FORM_IDWT;
   const long mul = wav ? 8 : 1;
   const long nti = mul * b.nx[0] * b.nx[1] * b.nx[2], nto = mul * b.ny[0] * b.ny[1] * b.ny[2];
+  // sets of (x, y) rotated so that consecutive launches miss the MALL
+  const long set_bytes = (long)sizeof(R) * (nti + nto);
+  const int nsets = (int)std::min<long>(kTuneSetsMax, std::max<long>(1, (kMallBytes * 3 + set_bytes - 1) / std::max(1L, set_bytes)));
   R *x = nullptr, *y = nullptr;
   double *dot = nullptr, *sc = nullptr;
   size_t sb = form == FORM_SUM ? sum3d_scratch_bytes<R>(b, 1) : 0;
@@ -155,17 +165,22 @@ hipError_t tune_md(int form, int id, const Box3& b, int repeat, int* best, doubl
   hipError_t err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
   if (err == hipSuccess) err = hipEventCreate(&e0);
   if (err == hipSuccess) err = hipEventCreate(&e1);
-  if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&x), sizeof(R) * nti);
-  if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&y), sizeof(R) * nto);
+  if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&x), sizeof(R) * nti * nsets);
+  if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&y), sizeof(R) * nto * nsets);
   if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&dot), sizeof(double) * 4);
   if (err == hipSuccess && sb) err = hipMalloc(reinterpret_cast<void**>(&sc), sb);
-  if (err == hipSuccess) err = fill<R>(x, nti, 12345u, st);
+  if (err == hipSuccess) err = fill<R>(x, nti * nsets, 12345u, st);
+  int k = 0;  // launch counter: set k % nsets
+  auto run = [&](int v) {
+    const int s = k++ % nsets;
+    return launch_variant<R>(form, id, v, b, x + nti * s, y + nto * s, dot, sc, sb, st);
+  };
   const int nv = variant_count(form);
   times->assign(nv, std::numeric_limits<double>::infinity());
   int bv = -1;
   double bt = std::numeric_limits<double>::infinity();
   for (int v = 0; err == hipSuccess && v < nv; ++v) {
-    if (launch_variant<R>(form, id, v, b, x, y, dot, sc, sb, st) != hipSuccess) {
+    if (run(v) != hipSuccess) {
       (void)hipGetLastError();
       continue;  // the variant does not take this shape
     }
@@ -173,12 +188,10 @@ hipError_t tune_md(int form, int id, const Box3& b, int repeat, int* best, doubl
     // one average; the minimum of three such averages is kept
     double tmin = std::numeric_limits<double>::infinity();
     const int reps = std::max(1, repeat);
-    for (int w = 0; w < 2 && err == hipSuccess; ++w)
-      err = launch_variant<R>(form, id, v, b, x, y, dot, sc, sb, st);
+    for (int w = 0; w < 2 && err == hipSuccess; ++w) err = run(v);
     for (int r = 0; r < 3 && err == hipSuccess; ++r) {
       err = hipEventRecord(e0, st);
-      for (int k = 0; k < reps && err == hipSuccess; ++k)
-        err = launch_variant<R>(form, id, v, b, x, y, dot, sc, sb, st);
+      for (int r2 = 0; r2 < reps && err == hipSuccess; ++r2) err = run(v);
       if (err == hipSuccess) err = hipEventRecord(e1, st);
       if (err == hipSuccess) err = hipEventSynchronize(e1);
       float ms = 0.f;

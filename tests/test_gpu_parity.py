@@ -191,6 +191,56 @@ def test_wavelet_3d_and_round_trip(lc, bc):
         assert rel(back, s) < TOL64
 
 
+def test_combined_operators_match_compositions(lc, gpu):
+    """symicomb8 through the library against the library's own sym8 and
+    magic-filter operators (BigDFT/TestOperations.ipynb cells 8-19):
+    s1tor = MF o IDWT and rtos1 = DWT o IMF along one dim of a 3D array
+    (periodic), the grow / shrink pair along dim 0, and the 3D periodic
+    s1tor through the device API."""
+    torch = gpu
+    rng = np.random.default_rng(610)
+    n = [7, 10, 6]
+    n2 = [2 * v for v in n]
+    x = rng.uniform(-1, 1, int(np.prod(n2)))
+    # periodic, idim 1
+    w, comp, comb = np.zeros(x.size), np.zeros(x.size), np.zeros(x.size)
+    lc.Wavelet1d('recompose').run(3, 1, n, 0, n, n, 1, x, w, 1.0, 0.0)
+    lc.MagicFilter1d().run(3, 1, n2, 0, n2, n2, 1, w, comp, 1.0, 0.0, 0.0)
+    lc.Wavelet1d('recompose', combined=True).run(3, 1, n, 0, n, n, 1, x, comb, 1.0, 0.0)
+    assert rel(comb, comp) < TOL64
+    w, comp, comb = np.zeros(x.size), np.zeros(x.size), np.zeros(x.size)
+    lc.InverseMagicFilter1d().run(3, 1, n2, 0, n2, n2, 1, x, w, 1.0, 0.0, 0.0)
+    lc.Wavelet1d('decompose').run(3, 1, n, 0, n, n, 1, w, comp, 1.0, 0.0)
+    lc.Wavelet1d('decompose', combined=True).run(3, 1, n, 0, n, n, 1, x, comb, 1.0, 0.0)
+    assert rel(comb, comp) < TOL64
+    # grow then shrink along dim 0 (1D arrays)
+    m = 13
+    v = rng.uniform(-1, 1, 2 * m)
+    s0 = np.zeros(2 * (m + 7))
+    lc.Wavelet1d('recompose').run(1, 0, [m], 1, [m], [m + 7], 1, v, s0, 1.0, 0.0)
+    sr = np.zeros(2 * m + 29)
+    lc.MagicFilter1d().run(1, 0, [2 * (m + 7)], 1, [2 * (m + 7)], [2 * m + 29], 1, s0, sr, 1.0, 0.0, 0.0)
+    sr1 = np.zeros(2 * (m + 15))
+    lc.Wavelet1d('recompose', combined=True).run(1, 0, [m], 1, [m], [m + 15], 1, v, sr1, 1.0, 0.0)
+    assert rel(sr1[:2 * m + 29], sr) < TOL64 and sr1[-1] == 0.0
+    s0b = np.zeros(2 * (m + 7))
+    lc.InverseMagicFilter1d().run(1, 0, [2 * (m + 7)], -1, [2 * m + 29], [2 * (m + 7)], 1, sr, s0b,
+                                  1.0, 0.0, 0.0)
+    p = np.zeros(2 * m)
+    lc.Wavelet1d('decompose').run(1, 0, [m], -1, [m + 7], [m], 1, s0b, p, 1.0, 0.0)
+    p2 = np.zeros(2 * m)
+    lc.Wavelet1d('decompose', combined=True).run(1, 0, [m], -1, [m + 15], [m], 1, sr1, p2, 1.0, 0.0)
+    assert rel(p2, p) < TOL64
+    # 3D periodic s1tor on device tensors (three sweeps) vs MF3D o IDWT3D
+    xt = torch.from_numpy(x).cuda()
+    w, comp, comb = torch.empty_like(xt), torch.empty_like(xt), torch.empty_like(xt)
+    lc.wavelet_md(lc.WAV_SYM8, True, n, [0] * 3, n, n, 1, xt, w)
+    lc.conv_md(lc.FIL_MD, n2, [0] * 3, n2, n2, 1, w, comp)
+    lc.wavelet_md(lc.WAV_SYMICOMB8, True, n, [0] * 3, n, n, 1, xt, comb)
+    torch.cuda.synchronize()
+    assert rel(comb.cpu().numpy(), comp.cpu().numpy()) < TOL64
+
+
 def test_wavelet_round_trip_config3_shape(lc, gpu):
     """sym8 DWT then IDWT on n = (96, 96, 96) pairs, periodic: identity."""
     torch = gpu

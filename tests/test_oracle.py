@@ -300,3 +300,58 @@ def test_wavelet_vanishing_moments():
         O.s0s1_1d(O.SYM8, 0, [n], 0, [n], [n], 1, s, c)
         H = c[n:]
         assert np.abs(H[8:n - 8]).max() < 1e-12, p
+
+
+def test_combined_operators_match_compositions():
+    """symicomb8 against the sym8 transforms and the magic filters, as the
+    reference checks them (BigDFT/TestOperations.ipynb cells 8-19):
+    s1tor = MF o IDWT and rtos1 = DWT o IMF (periodic); under grow, s1tor
+    equals MF(grow) o IDWT(grow) on its first 2n + 29 values (its extent is
+    2 (n + 15), the last value zero); and rtos1(shrink) of that equals
+    DWT(shrink) o IMF(shrink) of the composed grow result."""
+    rng = np.random.default_rng(19)
+    n = 11
+    x = rng.uniform(-1, 1, 2 * n)
+    # periodic, 1D
+    w = np.zeros(2 * n)
+    O.s1s0_1d(O.SYM8, 0, [n], 0, [n], [n], 1, x, w)
+    comp = np.zeros(2 * n)
+    O.s0s0_1d(O.MD, 0, [2 * n], 0, [2 * n], [2 * n], 1, w, comp)
+    comb = np.zeros(2 * n)
+    O.s1s0_1d(O.SYMICOMB8, 0, [n], 0, [n], [n], 1, x, comb)
+    assert rel(comb, comp) < 1e-14
+    w = np.zeros(2 * n)
+    O.s0s0_1d(O.IMD, 0, [2 * n], 0, [2 * n], [2 * n], 1, x, w)
+    comp = np.zeros(2 * n)
+    O.s0s1_1d(O.SYM8, 0, [n], 0, [n], [n], 1, w, comp)
+    comb = np.zeros(2 * n)
+    O.s0s1_1d(O.SYMICOMB8, 0, [n], 0, [n], [n], 1, x, comb)
+    assert rel(comb, comp) < 1e-14
+    # grow: IDWT n -> n + 7 pairs, MF 2 (n + 7) -> 2 (n + 7) + 15 values
+    assert O.wavelet_extents(O.SYMICOMB8, 1, n) == (n, n + 15)
+    s0 = np.zeros(2 * (n + 7))
+    O.s1s0_1d(O.SYM8, 0, [n], 1, [n], [n + 7], 1, x, s0)
+    sr = np.zeros(2 * n + 29)
+    O.s0s0_1d(O.MD, 0, [2 * (n + 7)], 1, [2 * (n + 7)], [2 * n + 29], 1, s0, sr)
+    sr1 = np.zeros(2 * (n + 15))
+    O.s1s0_1d(O.SYMICOMB8, 0, [n], 1, [n], [n + 15], 1, x, sr1)
+    assert rel(sr1[:2 * n + 29], sr) < 1e-14 and sr1[-1] == 0.0
+    # shrink back: IMF 2n + 29 -> 2 (n + 7) values, DWT n + 7 -> n pairs
+    s0b = np.zeros(2 * (n + 7))
+    O.s0s0_1d(O.IMD, 0, [2 * (n + 7)], -1, [2 * n + 29], [2 * (n + 7)], 1, sr, s0b)
+    p = np.zeros(2 * n)
+    O.s0s1_1d(O.SYM8, 0, [n], -1, [n + 7], [n], 1, s0b, p)
+    p2 = np.zeros(2 * n)
+    O.s0s1_1d(O.SYMICOMB8, 0, [n], -1, [n + 15], [n], 1, sr1, p2)
+    assert rel(p2, p) < 1e-14
+    # 3D, periodic: the separable products compose dimension by dimension
+    n3 = [5, 7, 6]
+    v = rng.uniform(-1, 1, 8 * int(np.prod(n3)))
+    n2 = [2 * k for k in n3]
+    w = np.zeros(v.size)
+    O.s1s0_md(O.SYM8, n3, [0] * 3, n3, n3, 1, v, w)
+    comp = np.zeros(v.size)
+    O.s0s0_md(O.MD, n2, [0] * 3, n2, n2, 1, w, comp)
+    comb = np.zeros(v.size)
+    O.s1s0_md(O.SYMICOMB8, n3, [0] * 3, n3, n3, 1, v, comb)
+    assert rel(comb, comp) < 1e-14
