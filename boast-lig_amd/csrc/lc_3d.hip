@@ -19,7 +19,7 @@ namespace lc {
 
 // LIBCONV_DIRECT=1 selects the unfused kernels (one lane per output for the
 // sum, three sweeps for the product): a debugging aid, not a fast path.
-static bool direct_kernels() {
+bool direct_kernels() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("LIBCONV_DIRECT");

@@ -154,8 +154,11 @@ int wav_window(int wav) {
 }
 
 // required extents of the processed dim for window W (steps)
-int check_line(int bc, int n, int nx, int ny, int W, const char* who) {
-  if (!valid_bc(bc)) return fail(LC_ERR_ARG, "%s: bc %d not in {0, 1, -1}", who, bc);
+// free_ok: the caller runs on the single-pass 3D convolution kernels, which
+// also take bc 2 (free, output on the input box)
+int check_line(int bc, int n, int nx, int ny, int W, const char* who, bool free_ok = false) {
+  if (!valid_bc(bc) && !(free_ok && bc == FREE))
+    return fail(LC_ERR_ARG, "%s: bc %d not in {0, 1, -1%s}", who, bc, free_ok ? ", 2" : "");
   if (n < 1) return fail(LC_ERR_ARG, "%s: n = %d < 1", who, n);
   const int need_in = bc == -1 ? n + W - 1 : n;
   const int need_out = bc == 1 ? n + W - 1 : n;
@@ -277,8 +280,9 @@ int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, 
   if (!bc) return fail(LC_ERR_ARG, "%s: null bc", who);
   const int W = kind == KCONV ? conv_window(id) : wav_window(id);
   if (W < 0) return fail(LC_ERR_ARG, "%s: unknown operator id %d", who, id);
+  const bool free_ok = kind == KCONV && d == 3 && !direct_kernels();
   for (int i = 0; i < d; ++i) {
-    rc = check_line(bc[i], n[i], nx[i], ny[i], W, who);
+    rc = check_line(bc[i], n[i], nx[i], ny[i], W, who, free_ok);
     if (rc) return rc;
   }
   const int mul = kind == KCONV ? 1 : 2;
@@ -321,9 +325,10 @@ int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, 
     return LC_OK;
   }
   // single-pass 3D sym8 wavelet transform (symicomb8 runs as three sweeps)
-  if (d == 3 && kind != KCONV && id == WAV_SYM8 && !getenv("LIBCONV_DIRECT")) {
-    Box3 b;
-    for (int i = 0; i < 3; ++i) { b.n[i] = n[i]; b.bc[i] = bc[i]; b.nx[i] = nx[i]; b.ny[i] = ny[i]; }
+  Box3 wb;
+  for (int i = 0; i < 3 && d == 3; ++i) { wb.n[i] = n[i]; wb.bc[i] = bc[i]; wb.nx[i] = nx[i]; wb.ny[i] = ny[i]; }
+  if (d == 3 && kind != KCONV && id == WAV_SYM8 && wav_plane_fits(wb, sizeof(R)) && !getenv("LIBCONV_DIRECT")) {
+    const Box3& b = wb;
     hipError_t e = sepw3d<R>(kind, id, b, narr, ax.dev, ay.dev, (R)a, (R)a_y, st);
     if (e != hipSuccess) return hip_fail(e, who);
     if (a_x != 0.0) {
@@ -416,7 +421,7 @@ int run_1ds(int fil, int d, const int* n, const int* bc, const int* nx, const in
   double a3[3] = {0, 0, 0};
   for (int i = 0; i < 3; ++i) {
     if (i < d) {
-      rc = check_line(bc[i], n[i], nx[i], ny[i], W, who);
+      rc = check_line(bc[i], n[i], nx[i], ny[i], W, who, !direct_kernels());
       if (rc) return rc;
       b.n[i] = n[i]; b.bc[i] = bc[i]; b.nx[i] = nx[i]; b.ny[i] = ny[i];
       a3[i] = a[i];
@@ -484,6 +489,79 @@ long cost_1d(int d, int idim, const int* n, int narr, int per_step, int mul) {
   return (long)narr * n[idim] * ndat * per_step;
 }
 
+// ---------------------------------------------------------------- local Hamiltonian
+// rho += psir^2, then psir = pot psir (the step between the magic filter and
+// its inverse in libconv.f90:266-271), one point per lane
+template <typename R>
+__global__ __launch_bounds__(256) void pot_density(long n, R* __restrict__ psir, const R* __restrict__ pot,
+                                                   R* __restrict__ rho) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const R t = psir[i];
+    if (rho) rho[i] = fmar(t, t, rho[i]);
+    psir[i] = pot[i] * t;
+  }
+}
+
+struct HamDims {
+  int n[3];   // coarse grid (pairs)
+  int f[3];   // psi_fscf extent 2 (n + 7): the synthesis grown by 7 pairs
+  int r[3];   // psir extent 2 (n + 7) + 15: the magic filter grown by 15
+  long nf, nr, ng;  // points of psi_fscf, psir, psig
+};
+
+int ham_dims(const int* n, HamDims* h, const char* who) {
+  if (!n) return fail(LC_ERR_ARG, "%s: null n", who);
+  h->nf = h->nr = h->ng = 1;
+  for (int i = 0; i < 3; ++i) {
+    if (n[i] < 1) return fail(LC_ERR_ARG, "%s: n[%d] = %d < 1", who, i, n[i]);
+    h->n[i] = n[i];
+    h->f[i] = 2 * (n[i] + 7);
+    h->r[i] = h->f[i] + 15;
+    h->nf *= h->f[i];
+    h->nr *= h->r[i];
+    h->ng *= 2L * n[i];
+  }
+  return LC_OK;
+}
+
+// libconv.f90:238-295, device data on one stream:
+//   psi_fscf = IDWT_grow psig            (s1s0_md sym8, bc 1)
+//   psir     = MF_grow psi_fscf          (s0s0_md sym8_md, bc 1)
+//   rho += psir^2; psir = pot psir
+//   hpsi     = IMF_shrink psir           (s0s0_md sym8_mi, bc -1)
+//   hpsi    += sum_d a_d D2_d psi_fscf   (s0s0_1ds_dot sym8_d2, bc 2 "free, nogrow", a_y = 1),
+//              ekin[d] = <psi_fscf, a_d D2_d psi_fscf>
+//   hpsig    = DWT_shrink hpsi           (s0s1_md sym8, bc -1)
+template <typename R>
+int run_local_hamiltonian(const int* n, const double* a, const R* psig, R* hpsig, const R* pot, R* rho,
+                          R* work, double* ekin_dev, hipStream_t st) {
+  const char* who = "lc_local_hamiltonian";
+  HamDims h;
+  int rc = ham_dims(n, &h, who);
+  if (rc) return rc;
+  if (!a || !psig || !hpsig || !pot || !work) return fail(LC_ERR_ARG, "%s: null argument", who);
+  R* fscf = work;
+  R* psir = work + h.nf;
+  R* hpsi = psir + h.nr;
+  const int g3[3] = {1, 1, 1}, s3[3] = {-1, -1, -1}, f3[3] = {FREE, FREE, FREE};
+  const int nf7[3] = {n[0] + 7, n[1] + 7, n[2] + 7};
+  rc = run_md<R>(KIDWT, WAV_SYM8, 3, h.n, g3, h.n, nf7, 1, psig, fscf, 1.0, 0.0, 0.0, st, true);
+  if (rc) return rc;
+  rc = run_md<R>(KCONV, OP_MD, 3, h.f, g3, h.f, h.r, 1, fscf, psir, 1.0, 0.0, 0.0, st, true);
+  if (rc) return rc;
+  {
+    const unsigned nb = (unsigned)std::min<long>((h.nr + 255) / 256, 8L * 1024);
+    pot_density<R><<<nb, 256, 0, st>>>(h.nr, psir, pot, rho);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, who);
+  }
+  rc = run_md<R>(KCONV, OP_IMD, 3, h.f, s3, h.r, h.f, 1, psir, hpsi, 1.0, 0.0, 0.0, st, true);
+  if (rc) return rc;
+  rc = run_1ds<R>(OP_D2, 3, h.f, f3, h.f, h.f, 1, a, 0.0, 1.0, fscf, hpsi, nullptr, ekin_dev, st, true);
+  if (rc) return rc;
+  return run_md<R>(KDWT, WAV_SYM8, 3, h.n, s3, nf7, h.n, 1, hpsi, hpsig, 1.0, 0.0, 0.0, st, true);
+}
+
 }  // namespace
 
 namespace lc {
@@ -522,7 +600,7 @@ int lc_extent(int op, int bc, int n, int side) {
   else if (op >= 10 && op < 12) W = wav_window(op - 10);
   else if (op >= 20 && op < 22) W = wav_window(op - 20);
   else return -1;
-  if (!valid_bc(bc) || n < 1) return -1;
+  if (!(valid_bc(bc) || (bc == FREE && op < 4)) || n < 1) return -1;
   if (side == 0) return bc == -1 ? n + W - 1 : n;
   return bc == 1 ? n + W - 1 : n;
 }
@@ -854,7 +932,7 @@ int check_item(const lc_batch_item& u, int k) {
   const int W = wav ? wav_window(u.op) : conv_window(u.op);
   if (W < 0) return fail(LC_ERR_ARG, "%s: item %d has unknown operator %d", who, k, u.op);
   for (int i = 0; i < 3; ++i) {
-    int rc = check_line(u.bc[i], u.n[i], u.nx[i], u.ny[i], W, who);
+    int rc = check_line(u.bc[i], u.n[i], u.nx[i], u.ny[i], W, who, !wav);
     if (rc) return rc;
   }
   if (!u.x || !u.y || !is_device_ptr(u.x) || !is_device_ptr(u.y))
@@ -1189,7 +1267,7 @@ static int form_check(int form, int op, const int* n, const int* bc, const int* 
   if (W < 0) return fail(LC_ERR_ARG, "%s: operator %d not tunable for form %d", who, op, form);
   if (!n || !bc || !nx || !ny) return fail(LC_ERR_ARG, "%s: null size array", who);
   for (int i = 0; i < 3; ++i) {
-    int rc = check_line(bc[i], n[i], nx[i], ny[i], W, who);
+    int rc = check_line(bc[i], n[i], nx[i], ny[i], W, who, !wav);
     if (rc) return rc;
     b->n[i] = n[i]; b->bc[i] = bc[i]; b->nx[i] = nx[i]; b->ny[i] = ny[i];
   }
@@ -1244,6 +1322,23 @@ int lc_kernel_name(int prec, int form, int op, const int* n, const int* bc, cons
                                                                      : fused_kernel_name(prec, form, op, v, false);
   snprintf(buf, (size_t)len, "%s", s.c_str());
   return LC_OK;
+}
+
+long lc_local_hamiltonian_work(const int* n) {
+  HamDims h;
+  if (ham_dims(n, &h, "lc_local_hamiltonian_work")) return -1;
+  return 2 * h.nf + h.nr;
+}
+
+int lc_local_hamiltonian(int prec, const int* n, const double* a, const void* psig, void* hpsig,
+                         const void* pot, void* rho, void* work, double* ekin_dev, void* stream) {
+  if (prec == 8)
+    return run_local_hamiltonian<double>(n, a, (const double*)psig, (double*)hpsig, (const double*)pot,
+                                         (double*)rho, (double*)work, ekin_dev, hs(stream));
+  if (prec == 4)
+    return run_local_hamiltonian<float>(n, a, (const float*)psig, (float*)hpsig, (const float*)pot,
+                                        (float*)rho, (float*)work, ekin_dev, hs(stream));
+  return fail(LC_ERR_ARG, "lc_local_hamiltonian: prec %d not in {4, 8}", prec);
 }
 
 void lc_batch_destroy(lc_batch* plan) {

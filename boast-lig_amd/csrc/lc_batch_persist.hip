@@ -231,7 +231,7 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
       pi.body = PB_SUM_D2;
       tab[i] = conv_args<R, FD2, 16, 16>(it[i], true);
       pi.dot = it[i].dot;
-    } else if ((k == BK_DWT || k == BK_IDWT) && id == WAV_SYM8) {
+    } else if ((k == BK_DWT || k == BK_IDWT) && id == WAV_SYM8 && wav_plane_fits(it[i].b, sizeof(R))) {
       pi.body = k == BK_DWT ? PB_DWT : PB_IDWT;
       tab[i] = k == BK_DWT ? wav_args_p<R, Dwt<Sym8Dec>>(it[i]) : wav_args_p<R, Idwt<Sym8Rec>>(it[i]);
     } else {

@@ -105,17 +105,20 @@ __device__ __forceinline__ void sepw3d_body(const FusedArgs<R>& p, int bx, int b
     const int v1 = rr / TH, s1 = rr % TH;
     const int j0 = fsrc(p.d0, X0 + LO + s0), j1 = fsrc(p.d1, Y0 + LO + s1);
     const bool ok = e < NA && j0 >= 0 && j1 >= 0;
-    aoff[k] = ok ? (int)(in_elem<K>(j0, v0, p.d0.nx) + rs_in * in_elem<K>(j1, v1, p.d1.nx)) : -1;
+    aoff[k] = plane_boff<R>(ok ? (int)(in_elem<K>(j0, v0, p.d0.nx) + rs_in * in_elem<K>(j1, v1, p.d1.nx)) : -1);
     aslot[k] = e < NA ? rr * AROW + v0 * TWS + s0 : -1;
   }
+  // the two z-planes of input step j2 as buffer resources: zero padding is
+  // read as out-of-range (no branch, no select on the loaded values)
   auto load_step = [&](int j2, R (*dst)[NAT]) {
-    const int s2 = fsrc(p.d2, j2);
-    const R* b0 = xa + (s2 >= 0 ? ps_in * in_elem<K>(s2, 0, p.d2.nx) : 0);
-    const R* b1 = xa + (s2 >= 0 ? ps_in * in_elem<K>(s2, 1, p.d2.nx) : 0);
+    const int s2 = fsrc(p.d2, j2);  // uniform
+    const long np2 = s2 >= 0 ? ps_in : 0;
+    const __amdgpu_buffer_rsrc_t r0 = span_rsrc(xa + (s2 >= 0 ? ps_in * in_elem<K>(s2, 0, p.d2.nx) : 0), np2);
+    const __amdgpu_buffer_rsrc_t r1 = span_rsrc(xa + (s2 >= 0 ? ps_in * in_elem<K>(s2, 1, p.d2.nx) : 0), np2);
 #pragma unroll
     for (int k = 0; k < NAT; ++k) {
-      dst[0][k] = (s2 >= 0 && aoff[k] >= 0) ? b0[aoff[k]] : R(0);
-      dst[1][k] = (s2 >= 0 && aoff[k] >= 0) ? b1[aoff[k]] : R(0);
+      dst[0][k] = plane_ld<R>(r0, aoff[k]);
+      dst[1][k] = plane_ld<R>(r1, aoff[k]);
     }
   };
 

@@ -19,7 +19,10 @@
 
 namespace lc {
 
-enum Bc : int { PER = 0, GROW = 1, SHRINK = -1 };
+// FREE: BoundaryConditions::FREE ("discard", GenericConvolution2.rb:684-740):
+// input [0, n) zero outside, output [0, n), for the 3D convolution kernels
+// only (libconv.f90:245 'F' nogrow, the Kinetic step of the local Hamiltonian)
+enum Bc : int { PER = 0, GROW = 1, SHRINK = -1, FREE = 2 };
 
 #define LC_AS_D(v) v,
 #define LC_AS_F(v) v##f,

@@ -20,7 +20,7 @@ struct FDim {   // one dim of a fused 3D operator
 
 __device__ __forceinline__ int fsrc(const FDim& g, int j) {
   if (g.bc == PER) return pmod(j, g.n);
-  if (g.bc == GROW) return (j >= 0 && j < g.n) ? j : -1;
+  if (g.bc == GROW || g.bc == FREE) return (j >= 0 && j < g.n) ? j : -1;
   const int s = j - g.ox;
   return (s >= 0 && s < g.nx) ? s : -1;  // beyond the stored input: never used
 }
@@ -35,11 +35,17 @@ __device__ __forceinline__ int fsrc(const FDim& g, int j) {
 constexpr int kOffNone = int(0x80000000u);
 constexpr int kBufDword3 = 0x00020000;  // raw 32-bit data format of gfx9 buffers
 
+// `count` elements from `base` (count 0: every load reads zero)
+template <typename R>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t span_rsrc(const R* base, long count) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(base), (short)0, int(count * (long)sizeof(R)),
+                                           kBufDword3);
+}
+
+// input plane s2 of sxy elements (s2 < 0: outside the input, reads zero)
 template <typename R>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const R* xa, long sxy, int s2) {
-  const R* pl = xa + (s2 >= 0 ? (long)s2 * sxy : 0);
-  const int nrec = s2 >= 0 ? int(sxy * (long)sizeof(R)) : 0;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(pl), (short)0, nrec, kBufDword3);
+  return span_rsrc(xa + (s2 >= 0 ? (long)s2 * sxy : 0), s2 >= 0 ? sxy : 0);
 }
 
 // element at byte offset `boff` of the plane; kOffNone reads zero

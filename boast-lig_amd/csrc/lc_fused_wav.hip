@@ -95,6 +95,7 @@ static FusedArgs<R> wav_args(const Box3& b, int narr, const R* x, R* y, R a, R a
 
 template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
 static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
+  if (!wav_plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, BZ, AD>(b, narr, x, y, a, a_y);
   const long tiles = (long)p.tiles_x * p.tiles_y;
   // z chunks no shorter than 2 W steps
@@ -148,6 +149,8 @@ static hipError_t wav_stage_build(const BatchItem<R>* it, int count, BatchStage*
   std::vector<FusedArgs<R>> tab(count);
   std::vector<int> first(count + 1, 0);
   long tiles_all = 0;
+  for (int o = 0; o < count; ++o)
+    if (!wav_plane_fits(it[o].b, sizeof(R))) return hipErrorInvalidValue;
   for (int o = 0; o < count; ++o) {
     tab[o] = wav_args<R, Op, TX, TY, QX, BZ, AD>(it[o].b, 1, it[o].x, it[o].y, (R)it[o].a[0], (R)it[o].a_y);
     tiles_all += (long)tab[o].tiles_x * tab[o].tiles_y;

@@ -37,7 +37,7 @@ struct WDim {
 
 __device__ __forceinline__ int wsrc(const WDim& g, int j) {
   if (g.bc == PER) return pmod(j, g.n);
-  if (g.bc == GROW) return (j >= 0 && j < g.n) ? j : -1;
+  if (g.bc == GROW || g.bc == FREE) return (j >= 0 && j < g.n) ? j : -1;
   const int s = j - g.ox;
   return (s >= 0 && s < g.nx) ? s : -1;
 }

@@ -58,6 +58,8 @@ struct Box3 {
 // planes of at most kPlaneMax bytes
 constexpr long kPlaneMax = 0x7fffffffL;
 inline bool plane_fits(const Box3& b, int prec) { return (long)b.nx[0] * b.nx[1] * prec <= kPlaneMax; }
+// wavelet arrays hold 2 x 2 values per (x, y) step
+inline bool wav_plane_fits(const Box3& b, int prec) { return 4L * b.nx[0] * b.nx[1] * prec <= kPlaneMax; }
 
 // separable product y = a * F2 F1 F0 x (+ a_y y), fused single pass
 template <typename R>
@@ -65,6 +67,9 @@ hipError_t sep3d(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
                  double* scratch, size_t scratch_bytes);
 // kinetic-style sum y = sum_d a_d F_d x + a_x x + a_y y with per-dim
 // energies dot[d] = <x, a_d F_d x> (of the last array) when dot != null
+// LIBCONV_DIRECT=1: the unfused 3D kernels (a debugging aid)
+bool direct_kernels();
+
 template <typename R>
 hipError_t sum3d(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
                  double* dot, hipStream_t st, double* scratch, size_t scratch_bytes);

@@ -318,6 +318,25 @@ def wavelet_md(wav, inverse, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0, stream=N
                             narr, _ptr(x), _ptr(y), a, a_y, _stream(stream)))
 
 
+def local_hamiltonian_work(n):
+    """Elements of the work array lc_local_hamiltonian needs for n pairs per dim."""
+    L = lib()
+    L.lc_local_hamiltonian_work.restype = ctypes.c_long
+    w = L.lc_local_hamiltonian_work(_ints(n).ctypes.data_as(_IP))
+    check()
+    return int(w)
+
+
+def local_hamiltonian(n, a, psig, hpsig, pot, rho, work, ekin=None, stream=None):
+    """lc_local_hamiltonian on device tensors (libconv.f90:238-295): hpsig =
+    DWT o (IMF o pot o MF + sum_d a_d D2_d) o IDWT psig, rho += psir^2."""
+    n_ = _ints(n)
+    av = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    _rc(lib().lc_local_hamiltonian(_prec(psig), n_.ctypes.data_as(_IP), av.ctypes.data_as(_DP), _ptr(psig),
+                                   _ptr(hpsig), _ptr(pot), _ptr(rho), _ptr(work), _ptr(ekin),
+                                   _stream(stream)))
+
+
 FIL_MD, FIL_IMD, FIL_D2, FIL_D1 = 0, 1, 2, 3
 WAV_SYM8, WAV_SYMICOMB8 = 0, 1
 

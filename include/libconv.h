@@ -283,7 +283,8 @@ int lc_batch_stage_variant(const lc_batch *plan, int stage);
  * ConvolutionOperator1d#optimize, BigDFT/GenericConvolution2.rb:744-837,
  * :1223-1267): the compiled kernel variants of a 3D operator form (tile
  * shape, outputs per lane, window step, prefetch depth) each run on random
- * data of the given shape in three rounds of `repeat` back-to-back launches;
+ * data of the given shape (rotating over operand sets larger than the 256 MB
+ * last-level cache) in three rounds of `repeat` back-to-back launches;
  * the minimum round average per variant is kept and the fastest variant is
  * recorded for that exact shape, which later
  * calls of the shape then use.  times[] (may be NULL) receives the min time
@@ -301,6 +302,24 @@ int lc_variant_count(int form);
 /* name of the kernel a call of this shape launches (as profilers show it) */
 int lc_kernel_name(int prec, int form, int op, const int *n, const int *bc, const int *nx,
                    const int *ny, char *buf, int len);
+
+/* ------------------------------------------- local Hamiltonian (one orbital)
+ * The chain libconv.f90:238-295 sketches for BigDFT's free-BC wavelets, on
+ * device data, in stream order:
+ *   psi_fscf = IDWT_grow psig                     (n -> n + 7 pairs per dim)
+ *   psir     = MF_grow psi_fscf                   (2 (n + 7) -> 2 (n + 7) + 15)
+ *   rho     += psir^2;  psir = pot psir           (rho may be NULL)
+ *   hpsi     = IMF_shrink psir                    (back to 2 (n + 7))
+ *   hpsi    += sum_d a[d] D2_d psi_fscf           (bc 2: free, output on the input box)
+ *   hpsig    = DWT_shrink hpsi                    (n pairs)
+ * psig / hpsig: (2 n)^3 values in the band layout of lc_wavelet_md; pot and
+ * rho: (2 n + 29)^3 values; a[d] = -0.5 / h_d^2 on the fine grid; ekin (3
+ * FP64, device, may be NULL) = <psi_fscf, a_d D2_d psi_fscf>.  work: device
+ * array of lc_local_hamiltonian_work(n) elements of the precision (psi_fscf,
+ * psir, hpsi), owned by the caller as the reference's work arrays are.     */
+long lc_local_hamiltonian_work(const int *n);
+int lc_local_hamiltonian(int prec, const int *n, const double *a, const void *psig, void *hpsig,
+                         const void *pot, void *rho, void *work, double *ekin, void *stream);
 
 /* extent of the input (side 0) or output (side 1) along a dim for an
  * operator: op 0..3 = fil ids, 10 + wav = analysis, 20 + wav = synthesis */

@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 MD, IMD, D2, D1 = 0, 1, 2, 3
 SYM8, SYMICOMB8 = 0, 1
-PERIODIC, GROW, SHRINK = 0, 1, -1
+PERIODIC, GROW, SHRINK, FREE = 0, 1, -1, 2
 
 _lib = None
 
@@ -126,6 +126,31 @@ def s1s0_md(wav, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0):
     (n_, pn), (b_, pb), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(bc), _iarr(nx), _iarr(ny)
     _check(lib().lco_s1s0_md(wav, len(n), pn, pb, pnx, pny, narr, _darr(x), _darr(y), a, a_y),
            's1s0_md')
+
+
+def local_hamiltonian(n, a, psig, pot, rho=None):
+    """The local-Hamiltonian chain of BigDFT/libconv.f90:238-295 composed from
+    the functions above: IDWT grow, MF grow, rho += psir^2 and psir *= pot,
+    IMF shrink, plus the free-BC ('F' nogrow, bc 2) Kinetic sum with a_y = 1
+    and energies, DWT shrink.  Returns (hpsig, rho, ekin[3]); rho is updated
+    in place when given."""
+    n = list(n)
+    nf7 = [v + 7 for v in n]
+    f = [2 * v for v in nf7]
+    r = [v + 15 for v in f]
+    fscf = np.zeros(int(np.prod(f)))
+    s1s0_md(SYM8, n, [GROW] * 3, n, nf7, 1, psig, fscf)
+    psir = np.zeros(int(np.prod(r)))
+    s0s0_md(MD, f, [GROW] * 3, f, r, 1, fscf, psir)
+    if rho is not None:
+        rho += psir * psir
+    psir *= pot
+    hpsi = np.zeros(fscf.size)
+    s0s0_md(IMD, f, [SHRINK] * 3, r, f, 1, psir, hpsi)
+    ekin = s0s0_1ds(D2, f, [FREE] * 3, f, f, 1, a, 0.0, 1.0, fscf, hpsi)
+    hpsig = np.zeros(int(np.prod([2 * v for v in n])))
+    s0s1_md(SYM8, n, [SHRINK] * 3, nf7, n, 1, hpsi, hpsig)
+    return hpsig, rho, ekin
 
 
 # ---------------------------------------------------------------------------

@@ -28,7 +28,13 @@ static const double TAP_CHPP[] = {LC_TAPS_CHPP(AS_D)};
 static const double TAP_CLPG[] = {LC_TAPS_CLPG(AS_D)};
 static const double TAP_CHPG[] = {LC_TAPS_CHPG(AS_D)};
 
-enum { BC_PER = 0, BC_GROW = 1, BC_SHRINK = -1 };
+/* BC_FREE (2) is BoundaryConditions::FREE ("fd", discard) of
+ * GenericConvolution2.rb:684-740: input [0, n) zero outside, output [0, n)
+ * (the grow result restricted to the input box).  The generator lists it but
+ * does not emit it (CONDITIONS = [PERIODIC, GROW, SHRINK]); libconv.f90:245
+ * asks for it ('F', "free, nogrow") for the Kinetic step of the local
+ * Hamiltonian.  The oracle takes it for convolutions only. */
+enum { BC_PER = 0, BC_GROW = 1, BC_SHRINK = -1, BC_FREE = 2 };
 
 int lco_num_threads(void) {
 #ifdef _OPENMP
@@ -100,7 +106,7 @@ static double conv_line(const cfilter *cf, int bc, int n,
             double v;
             if (bc == BC_PER) {
                 v = x[pmod(j, n) * sx];
-            } else if (bc == BC_GROW) {
+            } else if (bc == BC_GROW || bc == BC_FREE) {
                 if (j < 0 || j >= n) continue; /* both sides clamped */
                 v = x[j * sx];
             } else {
@@ -250,10 +256,10 @@ int lco_s0s0_1d(int fil, int d, int idim, const int *n, int bc,
                 double a, double a_x, double a_y, double *dot) {
     cfilter cf;
     if (get_cfilter(fil, &cf) || check_common(d, idim, n, nx, ny, narr)) return -1;
-    if (bc != BC_PER && bc != BC_GROW && bc != BC_SHRINK) return -1;
+    if (bc != BC_PER && bc != BC_GROW && bc != BC_SHRINK && bc != BC_FREE) return -1;
     int L = cf.up - cf.lo + 1;
     int nn = n[idim];
-    if (bc == BC_PER && (nx[idim] < nn || ny[idim] < nn)) return -1;
+    if ((bc == BC_PER || bc == BC_FREE) && (nx[idim] < nn || ny[idim] < nn)) return -1;
     if (bc == BC_GROW && (nx[idim] < nn || ny[idim] < nn + L - 1)) return -1;
     if (bc == BC_SHRINK && (nx[idim] < nn + L - 1 || ny[idim] < nn)) return -1;
     long left = 1, right = 1;
@@ -438,7 +444,7 @@ int lco_s0s0_1ds(int fil, int d, const int *n, const int *bc,
     long sx[16], sy[16];
     for (int i = 0; i < d; ++i) {
         if (n[i] < 1) return -1;
-        if (bc[i] == BC_PER) {
+        if (bc[i] == BC_PER || bc[i] == BC_FREE) {
             if (nx[i] < n[i] || ny[i] < n[i]) return -1;
             o_lo[i] = 0; o_hi[i] = n[i] - 1;
         } else if (bc[i] == BC_GROW) {
@@ -499,7 +505,10 @@ int lco_s0s0_1ds(int fil, int d, const int *n, const int *bc,
                             long jj = (long)idx[k] + l;
                             long pos;
                             if (bc[k] == BC_PER) pos = pmod(jj, n[k]);
-                            else if (bc[k] == BC_GROW) { if (jj < 0 || jj >= n[k]) continue; pos = jj; }
+                            else if (bc[k] == BC_GROW || bc[k] == BC_FREE) {
+                                if (jj < 0 || jj >= n[k]) continue;
+                                pos = jj;
+                            }
                             else pos = jj - ox[k];
                             t = xa[base + pos * sx[k]] * cf.f[l - lo] + t;
                         }

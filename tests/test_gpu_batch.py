@@ -193,6 +193,56 @@ def test_batch_equals_single_calls_and_oracle(lc, gpu, dtype_name):
         assert rel(got[k][3].cpu().numpy(), rw) < TOL64
 
 
+def test_batch_free_bc_items(lc, gpu):
+    """bc 2 (free, output on the input box) in SEP and SUM items: the stage
+    launches and the persistent launch equal one call per item bit for bit,
+    and the oracle to 1e-13."""
+    torch = gpu
+    rng = np.random.default_rng(1050)
+    a_k = [-0.5 / 0.3 ** 2, -0.5 / 0.35 ** 2, -0.5 / 0.25 ** 2]
+    mf, kin, bufs = [], [], []
+    for _ in range(6):
+        s = [int(v) for v in rng.integers(8, 36, 3)]
+        bcs = [int(b) for b in rng.choice([2, 0], 3)]
+        x = torch.from_numpy(rng.uniform(-1, 1, int(np.prod(s)))).cuda()
+        y1 = torch.zeros(x.numel(), dtype=torch.float64, device='cuda')
+        y2 = torch.zeros(x.numel(), dtype=torch.float64, device='cuda')
+        dot = torch.zeros(3, dtype=torch.float64, device='cuda')
+        mf.append(dict(kind=0, op=0, n=s, bc=bcs, nx=s, ny=s, x=x, y=y1))
+        kin.append(dict(kind=1, op=2, n=s, bc=bcs, nx=s, ny=s, x=y1, y=y2, a=a_k, dot=dot))
+        bufs.append((x, y1, y2, dot))
+    items = mf + kin
+    plan = lc.Batch(items)
+    plan.run()
+    torch.cuda.synchronize()
+    got = [[b.clone() for b in (y1, y2, dot)] for (_, y1, y2, dot) in bufs]
+    for (_, y1, y2, dot) in bufs:
+        for b in (y1, y2, dot):
+            b.fill_(3.0)
+    for it in items:
+        if it['kind'] == 0:
+            lc.conv_md(it['op'], it['n'], it['bc'], it['nx'], it['ny'], 1, it['x'], it['y'])
+        else:
+            lc.conv_1ds(it['op'], it['n'], it['bc'], it['nx'], it['ny'], 1, it['a'], 0.0, 0.0,
+                        it['x'], it['y'], dot=it['dot'])
+    torch.cuda.synchronize()
+    for k, (x, y1, y2, dot) in enumerate(bufs):
+        assert torch.equal(got[k][0], y1) and torch.equal(got[k][1], y2), k
+        r1 = np.zeros(y1.numel())
+        O.s0s0_md(O.MD, mf[k]['n'], mf[k]['bc'], mf[k]['n'], mf[k]['n'], 1, x.cpu().numpy(), r1)
+        r2 = np.zeros(y2.numel())
+        rdot = O.s0s0_1ds(O.D2, kin[k]['n'], kin[k]['bc'], kin[k]['n'], kin[k]['n'], 1, a_k, 0.0, 0.0,
+                          r1, r2)
+        assert rel(got[k][0].cpu().numpy(), r1) < TOL64
+        assert rel(got[k][1].cpu().numpy(), r2) < TOL64
+        assert np.abs(got[k][2].cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
+    plan.run_persistent()
+    assert plan.persistent_info()['status'] == 0
+    for k, (_, y1, y2, dot) in enumerate(bufs):
+        assert torch.equal(got[k][0], y1) and torch.equal(got[k][1], y2), k
+    plan.close()
+
+
 def test_batch_graph_replays(lc, gpu):
     """Each replay of the captured plan matches an eager run on the same inputs."""
     torch = gpu

@@ -173,6 +173,54 @@ def test_kinetic_3d_with_energies(lc, bcs):
     assert np.abs(st['reference_return']['dot_in'] - rdot).max() <= 1e-12 * np.abs(rdot).max()
 
 
+@pytest.mark.parametrize('bcs', [(2, 2, 2), (2, 0, -1), (1, 2, 0)])
+def test_free_bc_3d_operators(lc, gpu, bcs):
+    """bc 2 (free, output on the input box; libconv.f90:245 'F' nogrow) in the
+    single-pass 3D kernels: MagicFilter product and Kinetic sum with
+    energies, a_x and a_y, through the drop-in names and the stream API."""
+    torch = gpu
+    rng = np.random.default_rng(700)
+    n = [19, 22, 17]
+    nx, ny = [], []
+    for i in range(3):
+        a, b = O.conv_extents(O.MD, bcs[i], n[i])
+        nx.append(a)
+        ny.append(b)
+    x = rng.uniform(-1, 1, int(np.prod(nx)))
+    y, ref = np.zeros(int(np.prod(ny))), np.zeros(int(np.prod(ny)))
+    lc.MagicFilter().run(3, n, list(bcs), nx, ny, 1, x, y, None, None)
+    O.s0s0_md(O.MD, n, list(bcs), nx, ny, 1, x, ref)
+    assert rel(y, ref) < TOL64
+    xt, yt = torch.from_numpy(x).cuda(), torch.zeros(ref.size, dtype=torch.float64, device='cuda')
+    lc.conv_md(lc.FIL_IMD, n, list(bcs), nx, ny, 1, xt, yt)
+    O.s0s0_md(O.IMD, n, list(bcs), nx, ny, 1, x, ref)
+    torch.cuda.synchronize()
+    assert rel(yt.cpu().numpy(), ref) < TOL64
+    kx = [v + 28 if b == -1 else v for v, b in zip(n, bcs)]
+    ky = [v + 28 if b == 1 else v for v, b in zip(n, bcs)]
+    x = rng.uniform(-1, 1, int(np.prod(kx)))
+    y0 = rng.uniform(-1, 1, int(np.prod(ky)))
+    a = [-0.5 / v ** 2 for v in (0.3, 0.35, 0.4)]
+    ax = 0.5 if kx == ky else 0.0
+    y, ref = y0.copy(), y0.copy()
+    st = lc.Kinetic(dot=True).run(3, n, list(bcs), x, y, None, None, a, ax, 0.25)
+    rdot = O.s0s0_1ds(O.D2, n, list(bcs), kx, ky, 1, a, ax, 0.25, x, ref)
+    assert rel(y, ref) < TOL64
+    assert np.abs(st['reference_return']['dot_in'] - rdot).max() <= 1e-12 * np.abs(rdot).max()
+
+
+def test_free_bc_rejected_where_no_kernel_takes_it(lc):
+    """1D sweeps and wavelets have no free form: a clear error, no launch."""
+    n = [12, 12, 12]
+    x, y = np.zeros(12 ** 3), np.zeros(12 ** 3)
+    with pytest.raises(lc.LibconvError):
+        lc.MagicFilter1d().run(3, 0, n, 2, n, n, 1, x, y, 1.0, 0.0, 0.0)
+        lc.check()
+    with pytest.raises(lc.LibconvError):
+        lc.Wavelet('decompose').run(3, [6, 6, 6], [2, 2, 2], np.zeros(12 ** 3), y, None, None)
+        lc.check()
+
+
 @pytest.mark.parametrize('bc', [0, 1, -1])
 def test_wavelet_3d_and_round_trip(lc, bc):
     rng = np.random.default_rng(500)
@@ -304,3 +352,63 @@ def test_stream_api_device_tensors(lc, gpu):
     assert rel(y1.cpu().numpy(), r1) < TOL64
     assert rel(y2.cpu().numpy(), r2) < TOL64
     assert np.abs(dot.cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
+
+
+# ------------------------------------------------------------ local Hamiltonian
+def _ham_inputs(torch, rng, n, dtype):
+    ng = int(np.prod([2 * v for v in n]))
+    nr = int(np.prod([2 * (v + 7) + 15 for v in n]))
+    psig = rng.uniform(-1, 1, ng)
+    pot = rng.uniform(-1, 1, nr)
+    rho = rng.uniform(0, 1, nr)
+    return psig, pot, rho
+
+
+@pytest.mark.parametrize('dtype_name', ['float64', 'float32'])
+def test_local_hamiltonian_matches_oracle(lc, gpu, dtype_name):
+    """lc_local_hamiltonian (libconv.f90:238-295) against the oracle's
+    composition: hpsig, the updated density and the three energies."""
+    torch = gpu
+    dtype = getattr(torch, dtype_name)
+    rng = np.random.default_rng(900)
+    n = [7, 9, 6]
+    a = [-0.5 / 0.2 ** 2, -0.5 / 0.25 ** 2, -0.5 / 0.3 ** 2]
+    psig, pot, rho0 = _ham_inputs(torch, rng, n, dtype)
+    cast = (lambda v: v.astype(np.float32).astype(np.float64)) if dtype_name == 'float32' else (lambda v: v)
+    ref_h, ref_rho, ref_e = O.local_hamiltonian(n, a, cast(psig), cast(pot), cast(rho0).copy())
+    dev = lambda v: torch.from_numpy(v).to('cuda', dtype)
+    tp, tpot, trho = dev(psig), dev(pot), dev(rho0)
+    th = torch.full_like(tp, 9.0)
+    work = torch.empty(lc.local_hamiltonian_work(n), dtype=dtype, device='cuda')
+    ekin = torch.zeros(3, dtype=torch.float64, device='cuda')
+    lc.local_hamiltonian(n, a, tp, th, tpot, trho, work, ekin)
+    torch.cuda.synchronize()
+    tol = TOL64 if dtype_name == 'float64' else 2e-5
+    assert rel(th.cpu().numpy(), ref_h) < tol
+    assert rel(trho.cpu().numpy(), ref_rho) < tol
+    assert np.abs(ekin.cpu().numpy() - ref_e).max() <= (1e-12 if dtype_name == 'float64' else 1e-5) * np.abs(ref_e).max()
+
+
+def test_local_hamiltonian_is_symmetric(lc, gpu):
+    """At a realistic orbital size (48 x 40 x 44 pairs): <phi, H psi> =
+    <H phi, psi> for a real potential, and with pot = 0, <psi, H psi> equals
+    the sum of the returned energies (the Kinetic term in the wavelet basis)."""
+    torch = gpu
+    rng = np.random.default_rng(901)
+    n = [48, 40, 44]
+    a = [-0.5 / 0.2 ** 2] * 3
+    psig, pot, _ = _ham_inputs(torch, rng, n, torch.float64)
+    phig = rng.uniform(-1, 1, psig.size)
+    dev = lambda v: torch.from_numpy(v).cuda()
+    work = torch.empty(lc.local_hamiltonian_work(n), dtype=torch.float64, device='cuda')
+    tpot = dev(pot)
+    hp, hf = torch.empty(psig.size, dtype=torch.float64, device='cuda'), torch.empty(psig.size, dtype=torch.float64, device='cuda')
+    tpsi, tphi = dev(psig), dev(phig)
+    lc.local_hamiltonian(n, a, tpsi, hp, tpot, None, work)
+    lc.local_hamiltonian(n, a, tphi, hf, tpot, None, work)
+    lhs, rhs = float(torch.dot(tphi, hp)), float(torch.dot(hf, tpsi))
+    assert abs(lhs - rhs) < 1e-11 * max(abs(lhs), 1.0)
+    ekin = torch.zeros(3, dtype=torch.float64, device='cuda')
+    lc.local_hamiltonian(n, a, tpsi, hp, torch.zeros_like(tpot), None, work, ekin)
+    e = float(ekin.sum())
+    assert abs(float(torch.dot(tpsi, hp)) - e) < 1e-11 * abs(e)

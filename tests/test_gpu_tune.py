@@ -26,14 +26,15 @@ def lc(gpu):
 
 
 @pytest.mark.parametrize('v', range(28))
-def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v):
+@pytest.mark.parametrize('bc', [(1, -1, 0), (2, 0, 2)])
+def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v, bc):
     torch = gpu
     rng = np.random.default_rng(1200 + v)
     n = [37, 21, 45]
-    bc = [1, -1, 0]
+    bc = list(bc)
     # MagicFilter, separable form
-    nx = [n[0], n[1] + 15, n[2]]
-    ny = [n[0] + 15, n[1], n[2]]
+    nx = [O.conv_extents(O.MD, b, k)[0] for b, k in zip(bc, n)]
+    ny = [O.conv_extents(O.MD, b, k)[1] for b, k in zip(bc, n)]
     x = rng.uniform(-1, 1, int(np.prod(nx)))
     ref = np.zeros(int(np.prod(ny)))
     O.s0s0_md(O.MD, n, bc, nx, ny, 1, x, ref)
@@ -47,8 +48,8 @@ def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v):
     if v >= lc.variant_count(lc.FORM_SUM):
         return
     # Kinetic, sum form with energies, a_x and a_y
-    nx = [n[0], n[1] + 28, n[2]]
-    ny = [n[0] + 28, n[1], n[2]]
+    nx = [O.conv_extents(O.D2, b, k)[0] for b, k in zip(bc, n)]
+    ny = [O.conv_extents(O.D2, b, k)[1] for b, k in zip(bc, n)]
     x = rng.uniform(-1, 1, int(np.prod(nx)))
     y0 = rng.uniform(-1, 1, int(np.prod(ny)))
     a = [-5.5, -4.0, -3.0]

@@ -355,3 +355,78 @@ def test_combined_operators_match_compositions():
     comb = np.zeros(v.size)
     O.s1s0_md(O.SYMICOMB8, n3, [0] * 3, n3, n3, 1, v, comb)
     assert rel(comb, comp) < 1e-14
+
+
+def test_free_bc_is_grow_restricted_to_the_input_box():
+    """BC 2 (BoundaryConditions::FREE, 'discard', GenericConvolution2.rb:684-740;
+    libconv.f90:245 'F' nogrow): output [0, n) of the zero-extended input,
+    i.e. the grow result (origin -up) restricted to [0, n).  Pinned against
+    the grow form, which the reference Fortran and brute force pin, in 1D
+    (every filter), in the 3D separable product and in the 3D sum form with
+    energies."""
+    rng = np.random.default_rng(31)
+    for fil in (O.MD, O.IMD, O.D2, O.D1):
+        L = O.FILTER_LEN[fil]
+        lo, _ = O.taps(fil)
+        up = lo + L - 1
+        n = 23
+        x = rng.uniform(-1, 1, n)
+        g = np.zeros(n + L - 1)
+        O.s0s0_1d(fil, 0, [n], O.GROW, [n], [n + L - 1], 1, x, g)
+        f = np.zeros(n)
+        O.s0s0_1d(fil, 0, [n], O.FREE, [n], [n], 1, x, f)
+        assert rel(f, g[up:up + n]) < 1e-15
+    # 3D separable product: free in dims 0 and 2, periodic in dim 1
+    n = [9, 8, 10]
+    x = rng.uniform(-1, 1, int(np.prod(n)))
+    ng = [n[0] + 15, n[1], n[2] + 15]
+    g = np.zeros(int(np.prod(ng)))
+    O.s0s0_md(O.MD, n, [1, 0, 1], n, ng, 1, x, g)
+    f = np.zeros(x.size)
+    O.s0s0_md(O.MD, n, [2, 0, 2], n, n, 1, x, f)
+    g3 = g.reshape(ng[::-1])[7:7 + n[2], :, 7:7 + n[0]]
+    assert rel(f, g3.reshape(-1)) < 1e-15
+    # 3D sum form with a_x, a_y and energies: free in every dim
+    ng = [v + 28 for v in n]
+    y0 = rng.uniform(-1, 1, int(np.prod(ng)))
+    a = [-2.0, -3.0, -4.0]
+    g = y0.copy()
+    dg = O.s0s0_1ds(O.D2, n, [1, 1, 1], n, ng, 1, a, 0.5, 0.0, x, g)
+    f = y0.reshape(ng[::-1])[14:14 + n[2], 14:14 + n[1], 14:14 + n[0]].reshape(-1).copy()
+    df = O.s0s0_1ds(O.D2, n, [2, 2, 2], n, n, 1, a, 0.5, 0.0, x, f)
+    gin = g.reshape(ng[::-1])[14:14 + n[2], 14:14 + n[1], 14:14 + n[0]].reshape(-1)
+    assert rel(f, gin) < 1e-15
+    # grow energies only count points with input: the same points as free
+    assert np.abs(np.asarray(df) - np.asarray(dg)).max() <= 1e-14 * np.abs(dg).max()
+
+
+def test_local_hamiltonian_chain_is_symmetric():
+    """The chain of libconv.f90:238-295 is H = DWT_s (IMF_s V MF_g + K_free)
+    IDWT_g.  DWT shrink and IDWT grow are adjoint (orthonormal wavelets), the
+    inverse magic filter is the magic filter's transpose (reversed taps,
+    shrink vs grow), and the free Kinetic sum is symmetric: for a real
+    potential, <phi, H psi> = <H phi, psi>.  A dropped or mis-oriented step
+    breaks this; rho receives psir^2 of the synthesised orbital."""
+    rng = np.random.default_rng(41)
+    n = [5, 6, 4]
+    ng = int(np.prod([2 * v for v in n]))
+    r = [2 * (v + 7) + 15 for v in n]
+    pot = rng.uniform(-1, 1, int(np.prod(r)))
+    a = [-0.5 / 0.2 ** 2, -0.5 / 0.25 ** 2, -0.5 / 0.3 ** 2]
+    psi, phi = rng.uniform(-1, 1, ng), rng.uniform(-1, 1, ng)
+    rho = np.zeros(pot.size)
+    hpsi, rho, ekin = O.local_hamiltonian(n, a, psi, pot, rho)
+    hphi, _, _ = O.local_hamiltonian(n, a, phi, pot)
+    lhs, rhs = float(phi @ hpsi), float(hphi @ psi)
+    assert abs(lhs - rhs) < 1e-12 * max(abs(lhs), 1.0)
+    # density: psir^2 of psir = MF_grow IDWT_grow psi
+    f = [2 * (v + 7) for v in n]
+    fscf = np.zeros(int(np.prod(f)))
+    O.s1s0_md(O.SYM8, n, [1] * 3, n, [v + 7 for v in n], 1, psi, fscf)
+    psir = np.zeros(pot.size)
+    O.s0s0_md(O.MD, f, [1] * 3, f, r, 1, fscf, psir)
+    assert rel(rho, psir * psir) < 1e-15
+    # kinetic energy of the orbital is <psi, K psi>: with pot = 0 the chain is
+    # the wavelet-basis Kinetic operator and <psi, H psi> = sum of the energies
+    h0, _, e0 = O.local_hamiltonian(n, a, psi, np.zeros(pot.size))
+    assert abs(float(psi @ h0) - float(np.sum(e0))) < 1e-12 * abs(float(np.sum(e0)))
