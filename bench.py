@@ -180,6 +180,41 @@ def run_dwt(torch, lc, dtype, steps, warmup, n=96):
             'algorithmic_bytes': 2 * op_bytes, 'round_trip_rel_err': err}
 
 
+def run_hamiltonian(torch, lc, dtype, steps, warmup, n=64):
+    """The local-Hamiltonian chain of one orbital (libconv.f90:238-295) on n^3
+    pairs: IDWT grow, MF grow with the fused potential / density step, IMF
+    shrink, free-BC Kinetic accumulate with energies, DWT shrink."""
+    nn = [n] * 3
+    ng, nf, nr = (2 * n) ** 3, (2 * (n + 7)) ** 3, (2 * (n + 7) + 15) ** 3
+    g = torch.Generator(device='cuda').manual_seed(7)
+    rnd = lambda k: torch.rand(k, dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+    psig, pot = rnd(ng), rnd(nr)
+    rho = torch.zeros(nr, dtype=dtype, device='cuda')
+    hpsig = torch.empty_like(psig)
+    work = torch.empty(lc.local_hamiltonian_work(nn), dtype=dtype, device='cuda')
+    ekin = torch.zeros(3, dtype=torch.float64, device='cuda')
+    a = [-0.5 / (H / 2) ** 2] * 3
+    st = torch.cuda.current_stream()
+    for _ in range(warmup):
+        lc.local_hamiltonian(nn, a, psig, hpsig, pot, rho, work, ekin, stream=st)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with quiet_host():
+        e0.record(st)
+        for _ in range(steps):
+            lc.local_hamiltonian(nn, a, psig, hpsig, pot, rho, work, ekin, stream=st)
+        e1.record(st)
+        torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    sz = 4 if dtype == torch.float32 else 8
+    # per step: IDWT (ng + nf), MF (nf + nr) with pot read and rho r/w (3 nr),
+    # IMF (nr + nf), Kinetic (nf in, hpsi read and written), DWT (nf + ng)
+    byts = sz * (2 * ng + 7 * nf + 5 * nr)
+    return {'pairs': nn, 'ms': round(ms, 5), 'GB_s': round(byts / (ms * 1e-3) / 1e9, 1),
+            'frac_hbm': round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), 'algorithmic_bytes': byts,
+            'fine_points_per_s_G': round(nf / (ms * 1e-3) / 1e9, 3)}
+
+
 def batch_orbitals(count=512, seed=5):
     """Orbital sizes ~U{24..96} per dim and BC ~U{periodic, grow, shrink}."""
     import numpy as np
@@ -411,6 +446,8 @@ def main():
     if not args.no_extra:
         out['dwt3d'] = run_dwt(torch, lc, torch.float64, args.steps, args.warmup)
         out['dwt3d']['fp32'] = run_dwt(torch, lc, torch.float32, args.steps, args.warmup)
+        out['hamiltonian'] = run_hamiltonian(torch, lc, torch.float64, args.steps, args.warmup)
+        out['hamiltonian']['fp32'] = run_hamiltonian(torch, lc, torch.float32, args.steps, args.warmup)
         out['batch'] = run_batch(torch, lc, torch.float64, reps=3)
         out['batch']['fp32'] = run_batch(torch, lc, torch.float32, reps=3)
     if not args.no_oracle:

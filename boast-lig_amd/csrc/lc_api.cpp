@@ -491,7 +491,8 @@ long cost_1d(int d, int idim, const int* n, int narr, int per_step, int mul) {
 
 // ---------------------------------------------------------------- local Hamiltonian
 // rho += psir^2, then psir = pot psir (the step between the magic filter and
-// its inverse in libconv.f90:266-271), one point per lane
+// its inverse in libconv.f90:266-271), one point per lane; the unfused form
+// of the F_POT store step (used with LIBCONV_DIRECT=1)
 template <typename R>
 __global__ __launch_bounds__(256) void pot_density(long n, R* __restrict__ psir, const R* __restrict__ pot,
                                                    R* __restrict__ rho) {
@@ -547,9 +548,15 @@ int run_local_hamiltonian(const int* n, const double* a, const R* psig, R* hpsig
   const int nf7[3] = {n[0] + 7, n[1] + 7, n[2] + 7};
   rc = run_md<R>(KIDWT, WAV_SYM8, 3, h.n, g3, h.n, nf7, 1, psig, fscf, 1.0, 0.0, 0.0, st, true);
   if (rc) return rc;
-  rc = run_md<R>(KCONV, OP_MD, 3, h.f, g3, h.f, h.r, 1, fscf, psir, 1.0, 0.0, 0.0, st, true);
-  if (rc) return rc;
-  {
+  if (!direct_kernels()) {
+    // MF grow with the potential step fused into its store (no extra pass over psir)
+    Box3 b;
+    for (int i = 0; i < 3; ++i) { b.n[i] = h.f[i]; b.bc[i] = 1; b.nx[i] = h.f[i]; b.ny[i] = h.r[i]; }
+    hipError_t e = sep3d_fused_run<R>(OP_MD, b, 1, fscf, psir, R(1), R(0), st, pot, rho);
+    if (e != hipSuccess) return hip_fail(e, who);
+  } else {
+    rc = run_md<R>(KCONV, OP_MD, 3, h.f, g3, h.f, h.r, 1, fscf, psir, 1.0, 0.0, 0.0, st, true);
+    if (rc) return rc;
     const unsigned nb = (unsigned)std::min<long>((h.nr + 255) / 256, 8L * 1024);
     pot_density<R><<<nb, 256, 0, st>>>(h.nr, psir, pot, rho);
     hipError_t e = hipGetLastError();

@@ -358,6 +358,7 @@ __device__ __forceinline__ void sep3d_body(const FusedArgs<R>& p, int bx, int by
             R* yp = ya + (X0 + tx - p.d0.oy) +
                     (long)p.d0.ny * ((Y0 + ty0 + c - p.d1.oy) + (long)p.d1.ny * (z - p.d2.oy));
             if (p.flags & F_AY) sum = fmar(p.a_y, *yp, sum);
+            if (p.flags & F_POT) sum = pot_step(p, yp - ya, sum);
             *yp = sum;
           }
         }

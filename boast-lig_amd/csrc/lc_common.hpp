@@ -187,6 +187,8 @@ __host__ __device__ __forceinline__ int pmod(int a, int b) {
 
 // Epilogue flags (the dispatcher's variant selection,
 // GenericConvolution2.rb:1650-1699 and :173-203)
-enum Flags : int { F_A = 1, F_AX = 2, F_AY = 4, F_DOT = 8 };
+// F_POT: after the epilogue, rho += t^2 and t = pot t (the potential step of
+// the local Hamiltonian fused into the magic filter's store)
+enum Flags : int { F_A = 1, F_AX = 2, F_AY = 4, F_DOT = 8, F_POT = 16 };
 
 }  // namespace lc

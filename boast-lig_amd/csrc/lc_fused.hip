@@ -50,14 +50,16 @@ template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
                          double* dot, hipStream_t st, double* sc, size_t sb);
 template <typename R, class F>
-hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st);
+hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                         const R* pot, R* rho);
 template <typename R, class F>
 hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count, BatchStage* s,
                          hipStream_t st);
 #define LC_FUSED_EXTERN(R, F)                                                                            \
   extern template hipError_t sum3d_variant<R, F>(int, const Box3&, int, const R*, R*, const double*, R, R, \
                                                  double*, hipStream_t, double*, size_t);                 \
-  extern template hipError_t sep3d_variant<R, F>(int, const Box3&, int, const R*, R*, R, R, hipStream_t); \
+  extern template hipError_t sep3d_variant<R, F>(int, const Box3&, int, const R*, R*, R, R, hipStream_t, \
+                                                 const R*, R*);                                          \
   extern template hipError_t fused_stage_t<R, F>(bool, int, const BatchItem<R>*, int, BatchStage*,       \
                                                  hipStream_t);
 LC_FUSED_EXTERN(double, FMD)
@@ -177,12 +179,12 @@ hipError_t sum3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* 
 
 template <typename R>
 hipError_t sep3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
-                         hipStream_t st) {
+                         hipStream_t st, const R* pot, R* rho) {
   switch (id) {
-    case OP_MD: return sep3d_variant<R, FMD>(v, b, narr, x, y, a, a_y, st);
-    case OP_IMD: return sep3d_variant<R, FIMD>(v, b, narr, x, y, a, a_y, st);
-    case OP_D2: return sep3d_variant<R, FD2>(v, b, narr, x, y, a, a_y, st);
-    case OP_D1: return sep3d_variant<R, FD1>(v, b, narr, x, y, a, a_y, st);
+    case OP_MD: return sep3d_variant<R, FMD>(v, b, narr, x, y, a, a_y, st, pot, rho);
+    case OP_IMD: return sep3d_variant<R, FIMD>(v, b, narr, x, y, a, a_y, st, pot, rho);
+    case OP_D2: return sep3d_variant<R, FD2>(v, b, narr, x, y, a, a_y, st, pot, rho);
+    case OP_D1: return sep3d_variant<R, FD1>(v, b, narr, x, y, a, a_y, st, pot, rho);
   }
   return hipErrorInvalidValue;
 }
@@ -195,9 +197,12 @@ hipError_t sum3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, co
 }
 
 template <typename R>
-hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
-  const int v = select_variant(sizeof(R), FORM_SEP, id, b);
-  return sep3d_fused_v<R>(v, id, b, narr, x, y, a, a_y, st);
+hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                           const R* pot, R* rho) {
+  int v = select_variant(sizeof(R), FORM_SEP, id, b);
+  if (pot && v >= 5 && v <= 7)  // the wave-specialised forms have no potential step
+    v = plane_fits(b, sizeof(R)) ? (sizeof(R) == 4 ? 25 : 17) : 12;
+  return sep3d_fused_v<R>(v, id, b, narr, x, y, a, a_y, st, pot, rho);
 }
 
 // ====================================================================== batches
@@ -270,16 +275,16 @@ template hipError_t sum3d_fused_run<double>(int, const Box3&, int, const double*
 template hipError_t sum3d_fused_run<float>(int, const Box3&, int, const float*, float*, const double*,
                                            float, float, double*, hipStream_t, double*, size_t);
 template hipError_t sep3d_fused_run<double>(int, const Box3&, int, const double*, double*, double, double,
-                                            hipStream_t);
+                                            hipStream_t, const double*, double*);
 template hipError_t sep3d_fused_run<float>(int, const Box3&, int, const float*, float*, float, float,
-                                           hipStream_t);
+                                           hipStream_t, const float*, float*);
 template hipError_t sum3d_fused_v<double>(int, int, const Box3&, int, const double*, double*, const double*,
                                           double, double, double*, hipStream_t, double*, size_t);
 template hipError_t sum3d_fused_v<float>(int, int, const Box3&, int, const float*, float*, const double*,
                                          float, float, double*, hipStream_t, double*, size_t);
 template hipError_t sep3d_fused_v<double>(int, int, const Box3&, int, const double*, double*, double,
-                                          double, hipStream_t);
+                                          double, hipStream_t, const double*, double*);
 template hipError_t sep3d_fused_v<float>(int, int, const Box3&, int, const float*, float*, float, float,
-                                         hipStream_t);
+                                         hipStream_t, const float*, float*);
 
 }  // namespace lc

@@ -74,7 +74,16 @@ struct FusedArgs {
   int flags;
   double* part;  // 3 sets of per-block energies (sum3d) or null
   long nparts;   // blocks per array = sets stride
+  const R* pot;  // F_POT: potential and density on the output box (one array's
+  R* rho;        // offsets, shared by all arrays); rho may be null
 };
+
+// the F_POT step on output value t at offset o of an output array
+template <typename R>
+__device__ __forceinline__ R pot_step(const FusedArgs<R>& p, long o, R t) {
+  if (p.rho) p.rho[o] = fmar(t, t, p.rho[o]);
+  return p.pot[o] * t;
+}
 
 // Block -> (tile, z chunk, array).  Workgroups are dispatched round-robin
 // over the 8 XCDs of the chip, each with its own L2: the linear block id is
@@ -128,12 +137,15 @@ inline FDim fdim(int n, int bc, int nx, int ny, int lo, int up) {
   return g;
 }
 
-// z chunk: enough blocks for ~2 per CU, chunks no shorter than 4 * filter length
+// z chunk: enough blocks for ~2 per CU, chunks no shorter than 4 * filter
+// length; a grid too small to give every CU a block that way takes chunks
+// down to 2 * filter length (a chunk repeats W - 1 planes of work, which
+// costs less than idle CUs)
 inline int pick_zchunk(long tiles, int nz, int narr, int w, int ncu) {
-  const long want = 2L * ncu;
-  long chunks = (want + tiles * narr - 1) / (tiles * narr);
-  const long maxc = std::max(1, nz / (4 * w));
-  chunks = std::min(chunks, maxc);
+  const long per = tiles * std::max(narr, 1);
+  long chunks = std::min((2L * ncu + per - 1) / per, (long)std::max(1, nz / (4 * w)));
+  if (chunks * per < ncu)
+    chunks = std::max(chunks, std::min((ncu + per - 1) / per, (long)std::max(1, nz / (2 * w))));
   chunks = std::max(chunks, 1L);
   return (int)((nz + chunks - 1) / chunks);
 }

@@ -103,7 +103,7 @@ hipError_t launch_sum3d(const Box3& b, int narr, const R* x, R* y, const double*
 
 template <typename R, class F, int TX, int TY, int CPT, int QX, int BZ, int AD, int WPE>
 hipError_t launch_sep3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y,
-                               hipStream_t st) {
+                               hipStream_t st, const R* pot, R* rho) {
   FusedArgs<R> p{};
   p.x = x; p.y = y;
   p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
@@ -118,7 +118,8 @@ hipError_t launch_sep3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y,
   p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   p.a0 = a; p.a_y = a_y;
-  p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0);
+  p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0) | (pot ? F_POT : 0);
+  p.pot = pot; p.rho = rho;
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
   sep3d_fused<R, F, TX, TY, CPT, QX, BZ, AD, WPE><<<grid, TX * TY / CPT, 0, st>>>(p);
@@ -216,7 +217,8 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
 }
 
 template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int ZC, int PD>
-hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
+hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                         const R* pot, R* rho) {
   if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p{};
   p.x = x; p.y = y;
@@ -240,7 +242,8 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
   }
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   p.a0 = a; p.a_y = a_y;
-  p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0);
+  p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0) | (pot ? F_POT : 0);
+  p.pot = pot; p.rho = rho;
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
   sep3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE, PD><<<grid, TX * TY, 0, st>>>(p);
@@ -307,18 +310,19 @@ hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const
 
 template <typename R, class F>
 hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
-                                hipStream_t st) {
+                                hipStream_t st, const R* pot, R* rho) {
   switch (v) {
 #define X(I, TX, TY, CPT, QX, BZ, AD, WPE) \
-  case I: return launch_sep3d<R, F, TX, TY, CPT, QX, BZ, AD, WPE>(b, narr, x, y, a, a_y, st);
+  case I: return launch_sep3d<R, F, TX, TY, CPT, QX, BZ, AD, WPE>(b, narr, x, y, a, a_y, st, pot, rho);
     LC_SEP_VARIANTS(X)
 #undef X
 #define X(I, TX, TY) \
-  case I: return launch_sep3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
+  case I:            \
+    return pot ? hipErrorInvalidValue : launch_sep3d_ws<R, F, TX, TY, 4, 4, 2>(b, narr, x, y, a, a_y, st, 4, F::N);
     LC_WS_VARIANTS(X)
 #undef X
 #define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PD) \
-  case I: return launch_sep3d2<R, F, TX, TY, QX, QY, QZ, WPE, ZC, PD>(b, narr, x, y, a, a_y, st);
+  case I: return launch_sep3d2<R, F, TX, TY, QX, QY, QZ, WPE, ZC, PD>(b, narr, x, y, a, a_y, st, pot, rho);
     LC_SEP2_VARIANTS(X)
 #undef X
   }
@@ -462,7 +466,7 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
 #define LC_FUSED_INSTANTIATE(R, F)                                                                      \
   template hipError_t sum3d_variant<R, F>(int, const Box3&, int, const R*, R*, const double*, R, R,    \
                                           double*, hipStream_t, double*, size_t);                      \
-  template hipError_t sep3d_variant<R, F>(int, const Box3&, int, const R*, R*, R, R, hipStream_t);     \
+  template hipError_t sep3d_variant<R, F>(int, const Box3&, int, const R*, R*, R, R, hipStream_t, const R*, R*);     \
   template hipError_t fused_stage_t<R, F>(bool, int, const BatchItem<R>*, int, BatchStage*, hipStream_t);
 
 }  // namespace lc

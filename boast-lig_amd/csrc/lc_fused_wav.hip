@@ -98,11 +98,9 @@ static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, 
   if (!wav_plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, BZ, AD>(b, narr, x, y, a, a_y);
   const long tiles = (long)p.tiles_x * p.tiles_y;
-  // z chunks no shorter than 2 W steps
-  const long want = 2L * num_cu();
-  long chunks = (want + tiles * std::max(narr, 1) - 1) / (tiles * std::max(narr, 1));
-  chunks = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / (2 * Op::W))));
-  p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
+  // z chunks no shorter than 2 W steps (W steps when the grid is too small
+  // to give every CU a block)
+  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), std::max(1, Op::W / 2), num_cu());
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);

@@ -81,7 +81,9 @@ template <typename R>
 hipError_t sum3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x,
                            R a_y, double* dot, hipStream_t st, double* scratch, size_t scratch_bytes);
 template <typename R>
-hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st);
+// pot (optional): the F_POT step, rho += t^2 and t = pot t, on every output
+hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                           const R* pot = nullptr, R* rho = nullptr);
 size_t fused_sum3d_parts(const Box3& b, int narr, int lo, int up, int w);
 
 // wave-specialised single-pass kernels (lc_fused_ws.hip); per_cu = target
@@ -151,7 +153,7 @@ hipError_t sum3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* 
                          R a_y, double* dot, hipStream_t st, double* scratch, size_t sb);
 template <typename R>
 hipError_t sep3d_fused_v(int v, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
-                         hipStream_t st);
+                         hipStream_t st, const R* pot = nullptr, R* rho = nullptr);
 template <typename R>
 hipError_t sepw3d_v(int v, int kind, int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
                     hipStream_t st);

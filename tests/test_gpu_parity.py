@@ -412,3 +412,24 @@ def test_local_hamiltonian_is_symmetric(lc, gpu):
     lc.local_hamiltonian(n, a, tpsi, hp, torch.zeros_like(tpot), None, work, ekin)
     e = float(ekin.sum())
     assert abs(float(torch.dot(tpsi, hp)) - e) < 1e-11 * abs(e)
+
+
+@pytest.mark.parametrize('v', [0, 6, 12, 17, 22, 25])
+def test_local_hamiltonian_potential_step_in_every_form(lc, gpu, monkeypatch, v):
+    """The potential / density step rides in the magic filter's store: the
+    single-plane (0, 12) and two-plane (17, 22, 25) separable kernels take
+    it; a wave-specialised choice (6) falls back to the two-plane default."""
+    torch = gpu
+    monkeypatch.setenv('LIBCONV_SEP3D_VARIANT', str(v))
+    rng = np.random.default_rng(950 + v)
+    n = [6, 8, 7]
+    a = [-0.5 / 0.2 ** 2, -0.5 / 0.25 ** 2, -0.5 / 0.3 ** 2]
+    psig, pot, rho0 = _ham_inputs(torch, rng, n, torch.float64)
+    ref_h, ref_rho, _ = O.local_hamiltonian(n, a, psig, pot, rho0.copy())
+    tp, tpot, trho = (torch.from_numpy(u).cuda() for u in (psig, pot, rho0))
+    th = torch.empty_like(tp)
+    work = torch.empty(lc.local_hamiltonian_work(n), dtype=torch.float64, device='cuda')
+    lc.local_hamiltonian(n, a, tp, th, tpot, trho, work)
+    torch.cuda.synchronize()
+    assert rel(th.cpu().numpy(), ref_h) < TOL64
+    assert rel(trho.cpu().numpy(), ref_rho) < TOL64
