@@ -50,14 +50,14 @@ struct PState {  // device; zeroed before every run
 // tile shapes of the bodies in the persistent kernel (256 lanes each)
 #define LC_P_SEP(R, F) sep3d_body<R, F, 32, 16, 2, 4, 4, 1>
 #define LC_P_SUM(R) sum3d2_body<R, FD2, 16, 16, 4, 4, 2, 0>
-#define LC_P_WAV(R, OP) sepw3d_body<R, OP, 16, 8, 4, 2, 2>
+#define LC_P_WAV(R, OP) sepw3d_body<R, OP, 16, 8, 4, 1, 1>
 
 template <typename R>
 constexpr size_t persist_lds_bytes() {
   return std::max(std::max(sep3d_lds_bytes<R, FMD, 32, 16, 2, 4, 4, 1>(),
                            sep3d_lds_bytes<R, FIMD, 32, 16, 2, 4, 4, 1>()),
                   std::max(Sum2Geo<R, FD2, 16, 16, 4, 4, 2>::lds_bytes(),
-                           sepw3d_lds_bytes<R, Dwt<Sym8Dec>, 16, 8, 4, 2, 2>()));
+                           sepw3d_lds_bytes<R, Dwt<Sym8Dec>, 16, 8, 4, 1, 1>()));
 }
 
 __global__ void persist_reset(PState* st, int* done, int nitems) {
@@ -67,7 +67,9 @@ __global__ void persist_reset(PState* st, int* done, int nitems) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void batch_persistent(const FusedArgs<R>* __restrict__ tab,
+// two waves per SIMD at least: the bodies' union would otherwise take more
+// than 256 VGPRs in FP64 (one block per CU)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_persistent(const FusedArgs<R>* __restrict__ tab,
                                                         const PItem* __restrict__ items,
                                                         const int* __restrict__ deps,
                                                         const int2* __restrict__ work,
