@@ -169,6 +169,40 @@ def combined(lp_str, cv_str, cv_center):
     return ['%s' % t for t in taps], -lo
 
 
+def poisson_table(nord):
+    """Finite-difference first-derivative weights of order nord (the Poisson
+    nabla filters, BigDFT/Poisson2.rb generate_filter): row k in [-m, m]
+    (m = nord / 2) holds the weights c(j, k), j in [-m, m], of the
+    (nord + 1)-point stencil on the nodes j - k around the output point, so
+    that sum_j c(j, k) f(j - k) = f'(0) for every polynomial f of degree
+    <= nord.  Row 0 is the centred stencil; rows k < 0 serve the first m
+    outputs of a non-periodic line (output i uses row i - m and inputs
+    0 .. 2m), rows k > 0 the last m (GenericConvolution2.rb:237-294).  The
+    weights are computed exactly (Lagrange) and emitted as quotients of
+    integers, so each rounds once to the storage precision."""
+    from fractions import Fraction
+    m = nord // 2
+    rows = []
+    for k in range(-m, m + 1):
+        nodes = [j - k for j in range(-m, m + 1)]
+        w = []
+        for a in nodes:
+            others = [b for b in nodes if b != a]
+            den = Fraction(1)
+            for b in others:
+                den *= a - b
+            num = Fraction(0)  # derivative at 0 of prod_{b != a} (x - b)
+            for c in others:
+                p = Fraction(1)
+                for b in others:
+                    if b != c:
+                        p *= -b
+                num += p
+            w.append(num / den)
+        rows.append(w)
+    return rows
+
+
 def main():
     here = os.path.dirname(os.path.abspath(__file__))
     lp = SYM8_LP
@@ -210,6 +244,17 @@ def main():
         for i, t in enumerate(taps):
             end = ' \\' if i + 1 < len(taps) else ''
             lines.append('  X(%s)%s' % (t, end))
+        lines.append('')
+    lines.append('/* Poisson nabla filters: LC_POISSON_<nord>(X) expands X(p, q) for the')
+    lines.append(' * weight p / q of row k, tap j, rows k = -m .. m outer, taps j = -m .. m')
+    lines.append(' * inner (m = nord / 2); see poisson_table() in gen_taps.py. */')
+    for nord in (2, 4, 6, 8, 16):
+        rows = poisson_table(nord)
+        lines.append('#define LC_POISSON_%d(X) \\' % nord)
+        flat = [w for r in rows for w in r]
+        for i, w in enumerate(flat):
+            end = ' \\' if i + 1 < len(flat) else ''
+            lines.append('  X(%d, %d)%s' % (w.numerator, w.denominator, end))
         lines.append('')
     lines.append('#endif /* LC_TAPS_H */')
     with open(os.path.join(here, 'lc_taps.h'), 'w') as f:

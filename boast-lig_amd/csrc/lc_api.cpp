@@ -59,7 +59,7 @@ struct Pool {
   size_t bytes = 0;
 };
 constexpr int kMaxDev = 16;
-constexpr int kSlots = 4;  // 0 scratch, 1 x staging, 2 y staging, 3 dot
+constexpr int kSlots = 6;  // 0 scratch, 1 x staging, 2 y staging, 3 dot, 4-5 extra outputs
 thread_local Pool g_pool[kMaxDev][kSlots];
 
 int pool_get(int slot, size_t bytes, void** out) {
@@ -567,6 +567,66 @@ int run_local_hamiltonian(const int* n, const double* a, const R* psig, R* hpsig
   rc = run_1ds<R>(OP_D2, 3, h.f, f3, h.f, h.f, 1, a, 0.0, 1.0, fscf, hpsi, nullptr, ekin_dev, st, true);
   if (rc) return rc;
   return run_md<R>(KDWT, WAV_SYM8, 3, h.n, s3, nf7, h.n, 1, hpsi, hpsig, 1.0, 0.0, 0.0, st, true);
+}
+
+// ---------------------------------------------------------------- Poisson
+// validation shared by the Poisson entry points (div_u_i_ref: the grid must
+// exceed the stencil along every dim; orders 2, 4, 6, 8, 16)
+int poisson_check(int nord, const int* n, const char* who) {
+  if (!poisson_order_ok(nord)) return fail(LC_ERR_ARG, "%s: nord %d not in {2, 4, 6, 8, 16}", who, nord);
+  if (!n) return fail(LC_ERR_ARG, "%s: null n", who);
+  for (int i = 0; i < 3; ++i)
+    if (n[i] < nord + 1) return fail(LC_ERR_ARG, "%s: n[%d] = %d < nord + 1 = %d", who, i, n[i], nord + 1);
+  return LC_OK;
+}
+
+enum PoissonOp { PO_1D, PO_DIV, PO_NABLA, PO_NABLA2 };
+
+// one Poisson operator; arrays may be host (staged, synchronous) or device
+// (device_api: in place, asynchronous).  in: u (3 n for PO_DIV); out0: du
+// (3 n for the nabla forms); out1: cc (PO_DIV, optional) or du2 (PO_NABLA2)
+template <typename R>
+int run_poisson(int op, int nord, int idim, int bc, int geocode, const int* n, const R* u, R* out0, R* out1,
+                double a, double a_y, const double* h, hipStream_t st, bool device_api) {
+  const char* who = op == PO_1D ? "Poisson_broker" : op == PO_DIV ? "div_u_i" : op == PO_NABLA ? "nabla_u"
+                                                                                            : "nabla_u_and_square";
+  int rc = poisson_check(nord, n, who);
+  if (rc) return rc;
+  if (op == PO_1D && (idim < 0 || idim > 2)) return fail(LC_ERR_ARG, "%s: idim %d not in [0, 2]", who, idim);
+  if (op == PO_1D && bc != 0 && bc != -2) return fail(LC_ERR_ARG, "%s: bc %d not in {0, -2}", who, bc);
+  if (op != PO_1D && !h) return fail(LC_ERR_ARG, "%s: null hgrids", who);
+  if (!u || !out0 || (op == PO_NABLA2 && !out1)) return fail(LC_ERR_ARG, "%s: null array", who);
+  const size_t np = (size_t)n[0] * n[1] * n[2];
+  Arg<R> au, a0, a1;
+  if (device_api) {
+    au.dev = const_cast<R*>(u);
+    a0.dev = out0;
+    a1.dev = out1;
+  } else {
+    rc = stage_in(u, (op == PO_DIV ? 3 : 1) * np, 1, true, st, &au);
+    if (rc) return rc;
+    rc = stage_in<R>(out0, (op == PO_NABLA || op == PO_NABLA2 ? 3 : 1) * np, 2, op == PO_1D && a_y != 0.0, st,
+                     &a0);
+    if (rc) return rc;
+    if (out1) {
+      rc = stage_in<R>(out1, np, 4, false, st, &a1);
+      if (rc) return rc;
+    }
+  }
+  hipError_t e;
+  if (op == PO_1D)
+    e = lc::poisson_1d<R>(nord, idim, n, bc, au.dev, a0.dev, (R)a, (R)a_y, st);
+  else if (op == PO_DIV)
+    e = lc::div_u_i<R>(geocode, n, au.dev, a0.dev, a1.dev, nord, h, st);
+  else
+    e = lc::nabla_u<R>(geocode, n, au.dev, a0.dev, op == PO_NABLA2 ? a1.dev : nullptr, nord, h, st);
+  if (e != hipSuccess) return hip_fail(e, who);
+  if (!device_api) {
+    if (a0.host_in) HIPCHK(hipMemcpyAsync(out0, a0.dev, a0.count * sizeof(R), hipMemcpyDeviceToHost, st));
+    if (out1 && a1.host_in) HIPCHK(hipMemcpyAsync(out1, a1.dev, a1.count * sizeof(R), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return LC_OK;
 }
 
 }  // namespace
@@ -1329,6 +1389,69 @@ int lc_kernel_name(int prec, int form, int op, const int* n, const int* bc, cons
                                                                      : fused_kernel_name(prec, form, op, v, false);
   snprintf(buf, (size_t)len, "%s", s.c_str());
   return LC_OK;
+}
+
+// ---- Poisson nabla operators (BigDFT/Poisson2.rb)
+int lc_poisson_1d(int prec, int nord, int idim, const int* n, int bc, const void* u, void* du, double a,
+                  double a_y, void* stream) {
+  if (prec == 8)
+    return run_poisson<double>(PO_1D, nord, idim, bc, 0, n, (const double*)u, (double*)du, nullptr, a, a_y,
+                               nullptr, hs(stream), true);
+  if (prec == 4)
+    return run_poisson<float>(PO_1D, nord, idim, bc, 0, n, (const float*)u, (float*)du, nullptr, a, a_y,
+                              nullptr, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_poisson_1d: prec %d not in {4, 8}", prec);
+}
+int lc_div_u_i(int prec, int geocode, const int* n, const void* u, void* du, int nord, const double* h, void* cc,
+               void* stream) {
+  if (prec == 8)
+    return run_poisson<double>(PO_DIV, nord, -1, 0, geocode, n, (const double*)u, (double*)du, (double*)cc, 1.0,
+                               0.0, h, hs(stream), true);
+  if (prec == 4)
+    return run_poisson<float>(PO_DIV, nord, -1, 0, geocode, n, (const float*)u, (float*)du, (float*)cc, 1.0,
+                              0.0, h, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_div_u_i: prec %d not in {4, 8}", prec);
+}
+int lc_nabla_u(int prec, int geocode, const int* n, const void* u, void* du, int nord, const double* h,
+               void* stream) {
+  if (prec == 8)
+    return run_poisson<double>(PO_NABLA, nord, -1, 0, geocode, n, (const double*)u, (double*)du, nullptr, 1.0,
+                               0.0, h, hs(stream), true);
+  if (prec == 4)
+    return run_poisson<float>(PO_NABLA, nord, -1, 0, geocode, n, (const float*)u, (float*)du, nullptr, 1.0, 0.0,
+                              h, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_nabla_u: prec %d not in {4, 8}", prec);
+}
+int lc_nabla_u_and_square(int prec, int geocode, const int* n, const void* u, void* du, void* du2, int nord,
+                          const double* h, void* stream) {
+  if (prec == 8)
+    return run_poisson<double>(PO_NABLA2, nord, -1, 0, geocode, n, (const double*)u, (double*)du, (double*)du2,
+                               1.0, 0.0, h, hs(stream), true);
+  if (prec == 4)
+    return run_poisson<float>(PO_NABLA2, nord, -1, 0, geocode, n, (const float*)u, (float*)du, (float*)du2, 1.0,
+                              0.0, h, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_nabla_u_and_square: prec %d not in {4, 8}", prec);
+}
+
+// the procedure names the BOAST Poisson generator emits (C flavour, FP64,
+// host or device arrays): Poisson_broker (Poisson2.rb:1951-2043), div_u_i
+// (:2067-2195), nabla_u (:2553-2620), nabla_u_and_square (:2197-2276)
+void Poisson_broker(int nord, int idim, const int* nn, int bc, const double* u, double* du, double a) {
+  run_poisson<double>(PO_1D, nord, idim, bc, 0, nn, u, du, nullptr, a, 0.0, nullptr, nullptr, false);
+}
+void div_u_i(int geocode, int n01, int n02, int n03, const double* u, double* du, int nord, const double* hgrids,
+             double* cc) {
+  const int n[3] = {n01, n02, n03};
+  run_poisson<double>(PO_DIV, nord, -1, 0, geocode, n, u, du, cc, 1.0, 0.0, hgrids, nullptr, false);
+}
+void nabla_u(int geocode, int n01, int n02, int n03, const double* u, double* du, int nord, const double* hgrids) {
+  const int n[3] = {n01, n02, n03};
+  run_poisson<double>(PO_NABLA, nord, -1, 0, geocode, n, u, du, nullptr, 1.0, 0.0, hgrids, nullptr, false);
+}
+void nabla_u_and_square(int geocode, int n01, int n02, int n03, const double* u, double* du, double* du2, int nord,
+                        const double* hgrids) {
+  const int n[3] = {n01, n02, n03};
+  run_poisson<double>(PO_NABLA2, nord, -1, 0, geocode, n, u, du, du2, 1.0, 0.0, hgrids, nullptr, false);
 }
 
 long lc_local_hamiltonian_work(const int* n) {

@@ -70,6 +70,15 @@ hipError_t sep3d(int id, const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
 // LIBCONV_DIRECT=1: the unfused 3D kernels (a debugging aid)
 bool direct_kernels();
 
+// Poisson nabla operators (lc_poisson.hip); n[3], nord in {2, 4, 6, 8, 16}
+bool poisson_order_ok(int nord);
+template <typename R>
+hipError_t poisson_1d(int nord, int idim, const int* n, int bc, const R* u, R* du, R a, R a_y, hipStream_t st);
+template <typename R>
+hipError_t div_u_i(int geocode, const int* n, const R* u, R* du, R* cc, int nord, const double* h, hipStream_t st);
+template <typename R>
+hipError_t nabla_u(int geocode, const int* n, const R* u, R* du, R* du2, int nord, const double* h, hipStream_t st);
+
 template <typename R>
 hipError_t sum3d(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
                  double* dot, hipStream_t st, double* scratch, size_t scratch_bytes);

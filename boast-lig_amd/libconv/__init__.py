@@ -337,6 +337,38 @@ def local_hamiltonian(n, a, psig, hpsig, pot, rho, work, ekin=None, stream=None)
                                    _stream(stream)))
 
 
+# ------------------------------------------------------------------ Poisson
+NPERIODIC = -2
+
+
+def poisson_1d(nord, idim, n, bc, u, du, a=1.0, a_y=0.0, stream=None):
+    """lc_poisson_1d (Poisson_broker) on device tensors: du = a D_idim u (+ a_y du)."""
+    _rc(lib().lc_poisson_1d(_prec(u), nord, idim, _ints(n).ctypes.data_as(_IP), bc, _ptr(u), _ptr(du),
+                            _D(a), _D(a_y), _stream(stream)))
+
+
+def _h(h):
+    return np.ascontiguousarray(np.asarray(h, dtype=np.float64))
+
+
+def div_u_i(geocode, n, u, du, nord, h, cc=None, stream=None):
+    hv = _h(h)
+    _rc(lib().lc_div_u_i(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(du), nord,
+                         hv.ctypes.data_as(_DP), _ptr(cc), _stream(stream)))
+
+
+def nabla_u(geocode, n, u, du, nord, h, stream=None):
+    hv = _h(h)
+    _rc(lib().lc_nabla_u(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(du), nord,
+                         hv.ctypes.data_as(_DP), _stream(stream)))
+
+
+def nabla_u_and_square(geocode, n, u, du, du2, nord, h, stream=None):
+    hv = _h(h)
+    _rc(lib().lc_nabla_u_and_square(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(du),
+                                    _ptr(du2), nord, hv.ctypes.data_as(_DP), _stream(stream)))
+
+
 FIL_MD, FIL_IMD, FIL_D2, FIL_D1 = 0, 1, 2, 3
 WAV_SYM8, WAV_SYMICOMB8 = 0, 1
 

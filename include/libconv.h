@@ -321,6 +321,41 @@ long lc_local_hamiltonian_work(const int *n);
 int lc_local_hamiltonian(int prec, const int *n, const double *a, const void *psig, void *hpsig,
                          const void *pot, void *rho, void *work, double *ekin, void *stream);
 
+/* ------------------------------------------- Poisson nabla operators
+ * First derivatives of order nord in {2, 4, 6, 8, 16} (BigDFT/Poisson2.rb,
+ * filters of GenericConvolution2.rb:237-294 PoissonFilter) on a 3D grid n[3]
+ * (every n[k] > nord).  bc 0 periodic; bc -2 non-periodic: the first and
+ * last nord/2 outputs of a line use one-sided stencils.  geocode (div_u_i_ref
+ * convention) 1: x, y, z free; 2: all periodic; other: x, z periodic, y free.
+ * Vector fields are (n0, n1, n2, 3) arrays, component k at offset k n0 n1 n2.
+ * The stencil weights are the exact Lagrange weights; the reference's table
+ * differs in one entry (nord 8, first output of a non-periodic line, where
+ * its decimal table carries a sign typo).
+ * Stream API (device memory, asynchronous, int return):                      */
+/* du = a D_idim u (+ a_y du)  (Poisson_broker) */
+int lc_poisson_1d(int prec, int nord, int idim, const int *n, int bc, const void *u, void *du, double a,
+                  double a_y, void *stream);
+/* du = sum_k D_k u_k / h_k; cc (may be NULL) = sum_k u_k^2 d_k u_k
+ *      + 2 (u_x u_y d_x u_y + u_x u_z d_x u_z + u_y u_z d_y u_z)            */
+int lc_div_u_i(int prec, int geocode, const int *n, const void *u, void *du, int nord, const double *h,
+               void *cc, void *stream);
+/* du(:, :, :, k) = D_k u / h_k */
+int lc_nabla_u(int prec, int geocode, const int *n, const void *u, void *du, int nord, const double *h,
+               void *stream);
+/* the gradient and du2 = |grad u|^2 */
+int lc_nabla_u_and_square(int prec, int geocode, const int *n, const void *u, void *du, void *du2, int nord,
+                          const double *h, void *stream);
+/* The generated procedure names (C flavour, FP64, host or device arrays,
+ * synchronous): Poisson2.rb:1951-2043 (Poisson_broker), :2067-2195 (div_u_i),
+ * :2553-2620 (nabla_u), :2197-2276 (nabla_u_and_square).                    */
+void Poisson_broker(int nord, int idim, const int *nn, int bc, const double *u, double *du, double a);
+void div_u_i(int geocode, int n01, int n02, int n03, const double *u, double *du, int nord,
+             const double *hgrids, double *cc);
+void nabla_u(int geocode, int n01, int n02, int n03, const double *u, double *du, int nord,
+             const double *hgrids);
+void nabla_u_and_square(int geocode, int n01, int n02, int n03, const double *u, double *du, double *du2,
+                        int nord, const double *hgrids);
+
 /* extent of the input (side 0) or output (side 1) along a dim for an
  * operator: op 0..3 = fil ids, 10 + wav = analysis, 20 + wav = synthesis */
 int lc_extent(int op, int bc, int n, int side);

@@ -45,8 +45,14 @@ def lib():
         L.lco_s0s1_md.argtypes = [i, i, ip, ip, ip, ip, i, dp, dp, d, d]
         L.lco_s1s0_md.argtypes = [i, i, ip, ip, ip, ip, i, dp, dp, d, d]
         L.lco_filter_taps.argtypes = [i, ip, dp, i]
+        L.lco_poisson_taps.argtypes = [i, dp]
+        L.lco_poisson_1d.argtypes = [i, i, ip, i, d, d, dp, dp]
+        L.lco_div_u_i.argtypes = [i, ip, dp, dp, i, dp, dp]
+        L.lco_nabla_u.argtypes = [i, ip, dp, dp, i, dp]
+        L.lco_nabla_u_and_square.argtypes = [i, ip, dp, dp, dp, i, dp]
         for f in ('lco_s0s0_1d', 'lco_s0s1_1d', 'lco_s1s0_1d', 'lco_s0s0_md', 'lco_s0s0_1ds',
-                  'lco_s0s1_md', 'lco_s1s0_md', 'lco_filter_taps', 'lco_num_threads'):
+                  'lco_s0s1_md', 'lco_s1s0_md', 'lco_filter_taps', 'lco_num_threads', 'lco_poisson_taps',
+                  'lco_poisson_1d', 'lco_div_u_i', 'lco_nabla_u', 'lco_nabla_u_and_square'):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -151,6 +157,44 @@ def local_hamiltonian(n, a, psig, pot, rho=None):
     hpsig = np.zeros(int(np.prod([2 * v for v in n])))
     s0s1_md(SYM8, n, [SHRINK] * 3, nf7, n, 1, hpsi, hpsig)
     return hpsig, rho, ekin
+
+
+# ---------------------------------------------------------------------------
+# Poisson nabla operators (BigDFT/Poisson2.rb); bc 0 periodic, -2 non-periodic
+NPERIODIC = -2
+POISSON_ORDERS = (2, 4, 6, 8, 16)
+
+
+def poisson_taps(nord):
+    """(nord+1, nord+1) table c[k + m, j + m] of row k, tap j."""
+    out = np.zeros((nord + 1) ** 2)
+    _check(lib().lco_poisson_taps(nord, _darr(out)), 'poisson_taps')
+    return out.reshape(nord + 1, nord + 1)
+
+
+def poisson_1d(nord, idim, n, bc, u, du, a=1.0, a_y=0.0):
+    n_, pn = _iarr(n)
+    _check(lib().lco_poisson_1d(nord, idim, pn, bc, a, a_y, _darr(u), _darr(du)), 'poisson_1d')
+
+
+def div_u_i(geocode, n, u, du, nord, h, cc=None):
+    n_, pn = _iarr(n)
+    hv = np.ascontiguousarray(np.asarray(h, dtype=np.float64))
+    _check(lib().lco_div_u_i(geocode, pn, _darr(u), _darr(du), nord, _darr(hv),
+                             _darr(cc) if cc is not None else None), 'div_u_i')
+
+
+def nabla_u(geocode, n, u, du, nord, h):
+    n_, pn = _iarr(n)
+    hv = np.ascontiguousarray(np.asarray(h, dtype=np.float64))
+    _check(lib().lco_nabla_u(geocode, pn, _darr(u), _darr(du), nord, _darr(hv)), 'nabla_u')
+
+
+def nabla_u_and_square(geocode, n, u, du, du2, nord, h):
+    n_, pn = _iarr(n)
+    hv = np.ascontiguousarray(np.asarray(h, dtype=np.float64))
+    _check(lib().lco_nabla_u_and_square(geocode, pn, _darr(u), _darr(du), _darr(du2), nord, _darr(hv)),
+           'nabla_u_and_square')
 
 
 # ---------------------------------------------------------------------------

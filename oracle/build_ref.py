@@ -17,6 +17,9 @@ a trailing underscore, every argument by reference):
                   (the BOAST-generated FP64 IDWT family, cell 4 output)
   libref_wav.so   synthesis_{free,per}_ref,           BigDFT/old/Synthesis.rb:6-147
                   analysis_{free,per}_ref             BigDFT/old/AnaRotPer-2.rb:6-251
+  libref_poisson.so generate_filter, div_u_i_ref,     BigDFT/Poisson2.rb (heredocs)
+                  div_u_i_opt, nabla_u_ref (nabla_u_and_square_ref
+                  needs BigDFT's f_err_throw: not built)
 """
 import argparse
 import json
@@ -103,6 +106,12 @@ def main():
                     ('anal_f.f90', heredoc_after(ana, 'function_name = "analysis_free_ref"')),
                     ('anal_p.f90', heredoc_after(ana, 'function_name = "analysis_per_ref"'))],
                    os.path.join(args.out, 'libref_wav.so'), tmp)
+        poi = os.path.join(bd, 'Poisson2.rb')
+        # nabla_u_and_square_ref calls BigDFT's f_err_throw, which the
+        # reference does not carry: that routine is not built here
+        compile_so([('poi_%s.f90' % f, heredoc_after(poi, 'function_name = "%s"' % f))
+                    for f in ('generate_filter', 'div_u_i_ref', 'div_u_i_opt', 'nabla_u_ref')],
+                   os.path.join(args.out, 'libref_poisson.so'), tmp)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     print('build_ref: built', sorted(os.listdir(args.out)))

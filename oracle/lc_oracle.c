@@ -536,3 +536,176 @@ int lco_s0s0_1ds(int fil, int d, const int *n, const int *bc,
         for (int k = 0; k < d; ++k) dot[k] = last[k];
     return 0;
 }
+
+
+/* ------------------------------------------------------------------ Poisson
+ * First-derivative filters of order nord (BigDFT/Poisson2.rb generate_filter;
+ * GenericConvolution2.rb:237-294 PoissonFilter).  Row k of the table (k in
+ * [-m, m], m = nord / 2) weighs the inputs of the (nord + 1)-point stencil on
+ * the nodes j - k, j in [-m, m].  Along a line of n points:
+ *   periodic:      y[i] = sum_j c(j, 0) x[(i + j) mod n];
+ *   non-periodic:  i < m:       y[i] = sum_j c(j, i - m)         x[j + m],
+ *                  i >= n - m:  y[i] = sum_j c(j, i - n + m + 1) x[n - 1 - m + j],
+ *                  otherwise    y[i] = sum_j c(j, 0)             x[i + j]
+ * (div_u_i_ref, Poisson2.rb:183-335, and PoissonFilter#set_filter_val /
+ * #input_filter_index).  The weights are exact Lagrange weights (lc_taps.h,
+ * generated by gen_taps.py); they equal the reference's decimal table except
+ * c(0, -4) of nord 8, which the reference gives as +35/2 instead of -35/2. */
+#define PQ(p, q) ((double)(p) / (double)(q)),
+static const double PT2[] = {LC_POISSON_2(PQ)};
+static const double PT4[] = {LC_POISSON_4(PQ)};
+static const double PT6[] = {LC_POISSON_6(PQ)};
+static const double PT8[] = {LC_POISSON_8(PQ)};
+static const double PT16[] = {LC_POISSON_16(PQ)};
+#undef PQ
+
+static const double *poisson_table(int nord) {
+    switch (nord) {
+        case 2: return PT2;
+        case 4: return PT4;
+        case 6: return PT6;
+        case 8: return PT8;
+        case 16: return PT16;
+    }
+    return NULL;
+}
+
+int lco_poisson_taps(int nord, double *out) {
+    const double *t = poisson_table(nord);
+    if (!t) return -1;
+    memcpy(out, t, sizeof(double) * (nord + 1) * (nord + 1));
+    return 0;
+}
+
+/* derivative (unscaled) at point i of a line of n points, stride s */
+static double pderiv(const double *tab, int m, const double *base, long s, int n, int i, int per) {
+    const int w = 2 * m + 1;
+    double t = 0.0;
+    if (per) {
+        const double *c = tab + (long)m * w;
+        for (int j = -m; j <= m; ++j) t = t + c[j + m] * base[pmod((long)i + j, n) * s];
+    } else if (i < m) {
+        const double *c = tab + (long)i * w;              /* row i - m */
+        for (int j = -m; j <= m; ++j) t = t + c[j + m] * base[(long)(j + m) * s];
+    } else if (i >= n - m) {
+        const double *c = tab + (long)(i - n + 2 * m + 1) * w;  /* row i - n + m + 1 */
+        for (int j = -m; j <= m; ++j) t = t + c[j + m] * base[(long)(n - 1 - m + j) * s];
+    } else {
+        const double *c = tab + (long)m * w;
+        for (int j = -m; j <= m; ++j) t = t + c[j + m] * base[((long)i + j) * s];
+    }
+    return t;
+}
+
+static int poisson_dims_ok(int nord, const int *n) {
+    if (!poisson_table(nord) || !n) return 0;
+    for (int k = 0; k < 3; ++k)
+        if (n[k] < nord + 1) return 0;  /* div_u_i_ref: the grid exceeds the stencil */
+    return 1;
+}
+
+/* Poisson_broker (Poisson2.rb:1951-2043): du = a D_idim u (+ a_y du), idim
+ * 0-based, bc 0 or -2, on a 3D array n */
+int lco_poisson_1d(int nord, int idim, const int *n, int bc, double a, double a_y,
+                   const double *u, double *du) {
+    if (!poisson_dims_ok(nord, n) || idim < 0 || idim > 2 || (bc != 0 && bc != -2)) return -1;
+    const double *tab = poisson_table(nord);
+    const int m = nord / 2;
+    const long s[3] = {1, n[0], (long)n[0] * n[1]};
+    const long np = (long)n[0] * n[1] * n[2];
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < np; ++p) {
+        const int i[3] = {(int)(p % n[0]), (int)((p / n[0]) % n[1]), (int)(p / ((long)n[0] * n[1]))};
+        const double *base = u + (p - (long)i[idim] * s[idim]);
+        double t = a * pderiv(tab, m, base, s[idim], n[idim], i[idim], bc == 0);
+        if (a_y != 0.0) t = a_y * du[p] + t;
+        du[p] = t;
+    }
+    return 0;
+}
+
+static void geo_per(int geocode, int per[3]) {
+    per[0] = geocode != 1;
+    per[1] = geocode == 2;
+    per[2] = geocode != 1;
+}
+
+/* dd[k] = D_k f / h[k] at point p of f (division after the sum, as
+ * div_u_i_ref does) */
+static void grad_at(const double *tab, int m, const int per[3], const int *n, const double *f, long p,
+                    const double *h, int which, double dd[3]) {
+    const long s[3] = {1, n[0], (long)n[0] * n[1]};
+    const int i[3] = {(int)(p % n[0]), (int)((p / n[0]) % n[1]), (int)(p / ((long)n[0] * n[1]))};
+    for (int k = 0; k < 3; ++k) {
+        if (!(which & (1 << k))) continue;
+        const double *base = f + (p - (long)i[k] * s[k]);
+        dd[k] = pderiv(tab, m, base, s[k], n[k], i[k], per[k]) / h[k];
+    }
+}
+
+/* div_u_i (Poisson2.rb:183-335 div_u_i_ref; cc as div_u_i_opt :884-1247 and
+ * the BOAST composition :2067-2195):
+ *   du = d_x u_x + d_y u_y + d_z u_z,
+ *   cc = u_x^2 d_x u_x + u_y^2 d_y u_y + u_z^2 d_z u_z
+ *        + 2 (u_x u_y d_x u_y + u_x u_z d_x u_z + u_y u_z d_y u_z)   (if cc) */
+int lco_div_u_i(int geocode, const int *n, const double *u, double *du, int nord,
+                const double *h, double *cc) {
+    if (!poisson_dims_ok(nord, n) || !h) return -1;
+    const double *tab = poisson_table(nord);
+    const int m = nord / 2;
+    int per[3];
+    geo_per(geocode, per);
+    const long np = (long)n[0] * n[1] * n[2];
+    const double *u0 = u, *u1 = u + np, *u2 = u + 2 * np;
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < np; ++p) {
+        double a[3], b[3], c[3];
+        grad_at(tab, m, per, n, u0, p, h, 1, a);
+        grad_at(tab, m, per, n, u1, p, h, cc ? 3 : 2, b);
+        grad_at(tab, m, per, n, u2, p, h, cc ? 7 : 4, c);
+        du[p] = a[0] + b[1] + c[2];
+        if (cc) {
+            const double x = u0[p], y = u1[p], z = u2[p];
+            cc[p] = (x * x) * a[0] + (y * y) * b[1] + (z * z) * c[2] +
+                    2.0 * (x * y * b[0] + x * z * c[0] + y * z * c[1]);
+        }
+    }
+    return 0;
+}
+
+/* nabla_u (Poisson2.rb:722-874 nabla_u_ref): du(:, :, :, k) = D_k u / h_k */
+int lco_nabla_u(int geocode, const int *n, const double *u, double *du, int nord,
+                const double *h) {
+    if (!poisson_dims_ok(nord, n) || !h) return -1;
+    const double *tab = poisson_table(nord);
+    const int m = nord / 2;
+    int per[3];
+    geo_per(geocode, per);
+    const long np = (long)n[0] * n[1] * n[2];
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < np; ++p) {
+        double g[3];
+        grad_at(tab, m, per, n, u, p, h, 7, g);
+        du[p] = g[0];
+        du[np + p] = g[1];
+        du[2 * np + p] = g[2];
+    }
+    return 0;
+}
+
+/* nabla_u_and_square (Poisson2.rb:361-512): the gradient and
+ * du2 = |grad u|^2 (components squared and summed in order x, y, z) */
+int lco_nabla_u_and_square(int geocode, const int *n, const double *u, double *du,
+                           double *du2, int nord, const double *h) {
+    int rc = lco_nabla_u(geocode, n, u, du, nord, h);
+    if (rc) return rc;
+    const long np = (long)n[0] * n[1] * n[2];
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < np; ++p) {
+        double t = du[p] * du[p];
+        t = t + du[np + p] * du[np + p];
+        t = t + du[2 * np + p] * du[2 * np + p];
+        du2[p] = t;
+    }
+    return 0;
+}

@@ -79,6 +79,22 @@ int lco_filter_taps(int fil, int *lo, double *out, int cap);
 /* number of OpenMP threads the oracle runs with (1 without OpenMP). */
 int lco_num_threads(void);
 
+/* Poisson nabla operators (BigDFT/Poisson2.rb; filters of
+ * GenericConvolution2.rb:237-294).  nord in {2, 4, 6, 8, 16}; n[3] the
+ * grid; bc 0 periodic or -2 non-periodic (one-sided rows at the borders);
+ * geocode 1 free in x, y, z; 2 periodic in all; any other value periodic in
+ * x and z, free in y (div_u_i_ref's perx / pery / perz).  Vector fields are
+ * (n0, n1, n2, 3) arrays, component c at offset c n0 n1 n2. */
+int lco_poisson_taps(int nord, double *out);  /* (nord+1)^2 values, row k outer */
+int lco_poisson_1d(int nord, int idim, const int *n, int bc, double a, double a_y,
+                   const double *u, double *du);
+int lco_div_u_i(int geocode, const int *n, const double *u, double *du, int nord,
+                const double *h, double *cc);
+int lco_nabla_u(int geocode, const int *n, const double *u, double *du, int nord,
+                const double *h);
+int lco_nabla_u_and_square(int geocode, const int *n, const double *u, double *du,
+                           double *du2, int nord, const double *h);
+
 #ifdef __cplusplus
 }
 #endif

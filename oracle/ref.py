@@ -89,3 +89,42 @@ def bigdft_wavelet(kind, n, ndat, x):
     arg = n if kind == 'synthesis_per' else n - 1
     getattr(L, '%s_ref_' % kind)(_i(arg), _i(ndat), _p(x), _p(y))
     return y
+
+
+def poisson_filter(nord):
+    """generate_filter (BigDFT/Poisson2.rb:20-166): the (nord+1)^2 table
+    c1D(-m:m, -m:m), returned as [k + m, j + m] (row k, tap j)."""
+    L = _load('libref_poisson.so')
+    c = np.zeros((nord + 1, nord + 1), order='F')
+    L.generate_filter_(_i(nord), _p(c))
+    return np.ascontiguousarray(c.T)  # Fortran (j, k) -> [k][j]
+
+
+def _c1d(table):
+    return np.asfortranarray(np.asarray(table, dtype=np.float64).T)  # [k][j] -> Fortran (j, k)
+
+
+def div_u_i(geocode, n, u, nord, h, table, with_cc=False):
+    """div_u_i_ref / div_u_i_opt (BigDFT/Poisson2.rb:183-335, :901-1247) with
+    the given filter table; u is (n0, n1, n2, 3) Fortran order."""
+    L = _load('libref_poisson.so')
+    hv = np.asfortranarray(np.asarray(h, dtype=np.float64))
+    du = np.zeros(tuple(n), order='F')
+    if not with_cc:
+        L.div_u_i_ref_(_i(geocode), _i(n[0]), _i(n[1]), _i(n[2]), _p(u), _p(du), _i(nord), _p(hv),
+                       _p(_c1d(table)))
+        return du
+    cc = np.zeros(tuple(n), order='F')
+    L.div_u_i_opt_(_i(geocode), _i(n[0]), _i(n[1]), _i(n[2]), _p(u), _p(du), _i(nord), _p(hv),
+                   _p(_c1d(table)), _p(cc))
+    return du, cc
+
+
+def nabla_u(geocode, n, u, nord, h, table):
+    """nabla_u_ref (BigDFT/Poisson2.rb:722-874): du (n0, n1, n2, 3)."""
+    L = _load('libref_poisson.so')
+    hv = np.asfortranarray(np.asarray(h, dtype=np.float64))
+    du = np.zeros(tuple(n) + (3,), order='F')
+    L.nabla_u_ref_(_i(geocode), _i(n[0]), _i(n[1]), _i(n[2]), _p(u), _p(du), _i(nord), _p(hv),
+                   _p(_c1d(table)))
+    return du

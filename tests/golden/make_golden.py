@@ -14,6 +14,8 @@ that the reference's FP64 code produced for them:
   bigdft_wavelet.npz  BigDFT's own synthesis_per_ref / analysis_per_ref
                                                         BigDFT/old/Synthesis.rb,
                                                         BigDFT/old/AnaRotPer-2.rb
+  poisson.npz generate_filter tables (nord 2..16), div_u_i_ref, div_u_i_opt
+              (du, cc), nabla_u_ref                     BigDFT/Poisson2.rb
 
 Usage: python3 tests/golden/make_golden.py  (from the repository root).
 """
@@ -95,8 +97,36 @@ def bigdft_wavelet():
                         analysis_per=R.bigdft_wavelet('analysis_per', n, ndat, x))
 
 
+def poisson():
+    """generate_filter's tables for every order, and div_u_i_ref, div_u_i_opt
+    (du and cc) and nabla_u_ref on seeded vector fields, each with the
+    reference's own table; for nord 8 also with the exact Lagrange table
+    (the reference's c1D(0, -4) of nord 8 carries a sign typo)."""
+    from oracle import poisson_taps
+    rng = np.random.default_rng(505)
+    small, big = (11, 12, 13), (17, 17, 17)
+    us = np.asfortranarray(rng.uniform(-1, 1, small + (3,)))
+    ub = np.asfortranarray(rng.uniform(-1, 1, big + (3,)))
+    h = np.array([0.15, 0.2, 0.25])
+    out = dict(h=h, u_small=us, u_big=ub, n_small=np.array(small), n_big=np.array(big))
+    for nord in (2, 4, 6, 8, 16):
+        out['table_%d' % nord] = R.poisson_filter(nord)
+    cases = [(2, 1, 'ref'), (4, 3, 'ref'), (6, 2, 'ref'), (8, 1, 'ref'), (8, 1, 'lagrange'), (16, 3, 'ref')]
+    for nord, geo, tab in cases:
+        n = big if nord == 16 else small
+        u = ub if nord == 16 else us
+        table = R.poisson_filter(nord) if tab == 'ref' else poisson_taps(nord)
+        key = '%d_%d_%s' % (nord, geo, tab)
+        out['div_' + key] = R.div_u_i(geo, n, u, nord, h, table)
+        du, cc = R.div_u_i(geo, n, u, nord, h, table, with_cc=True)
+        out['divopt_' + key], out['cc_' + key] = du, cc
+        out['nabla_' + key] = R.nabla_u(geo, n, np.asfortranarray(u[..., 0]), nord, h, table)
+    out['cases'] = np.array(['%d_%d_%s' % c for c in cases])
+    np.savez_compressed(os.path.join(OUT, 'poisson.npz'), **out)
+
+
 def main():
-    for lib in ('libref_mf.so', 'libref_kin.so', 'libref_idwt.so', 'libref_wav.so'):
+    for lib in ('libref_mf.so', 'libref_kin.so', 'libref_idwt.so', 'libref_wav.so', 'libref_poisson.so'):
         if not R.available(lib):
             print('make_golden: oracle/_ref/%s missing (run make -C oracle ref)' % lib)
             return 1
@@ -104,6 +134,7 @@ def main():
     kinetic()
     idwt()
     bigdft_wavelet()
+    poisson()
     print('make_golden: wrote', sorted(f for f in os.listdir(OUT) if f.endswith('.npz')))
     return 0
 

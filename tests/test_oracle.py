@@ -430,3 +430,115 @@ def test_local_hamiltonian_chain_is_symmetric():
     # the wavelet-basis Kinetic operator and <psi, H psi> = sum of the energies
     h0, _, e0 = O.local_hamiltonian(n, a, psi, np.zeros(pot.size))
     assert abs(float(psi @ h0) - float(np.sum(e0))) < 1e-12 * abs(float(np.sum(e0)))
+
+
+# ------------------------------------------------------------------ Poisson
+def test_poisson_tables_match_reference_generate_filter(golden_dir):
+    """The oracle's (and the library's) finite-difference tables are exact
+    Lagrange weights.  They equal, bit for bit, the doubles of the
+    reference's decimal table (generate_filter, BigDFT/Poisson2.rb:20-166,
+    compiled and run into tests/golden/poisson.npz) except c1D(0, -4) of
+    nord 8, where the reference has +35/2 and the weight is -35/2."""
+    g = np.load(os.path.join(golden_dir, 'poisson.npz'))
+    for nord in O.POISSON_ORDERS:
+        ref, ours = g['table_%d' % nord], O.poisson_taps(nord)
+        diff = np.argwhere(ref != ours)
+        if nord == 8:
+            assert diff.tolist() == [[0, 4]]
+            assert ref[0, 4] == 17.5 and ours[0, 4] == -17.5
+        else:
+            assert diff.size == 0, (nord, diff)
+
+
+@pytest.mark.parametrize('nord', [2, 4, 6, 8, 16])
+def test_poisson_rows_exact_on_polynomials(nord):
+    """Every row k differentiates the polynomials of degree <= nord exactly on
+    its nodes j - k (the defining property of the weights)."""
+    m = nord // 2
+    t = O.poisson_taps(nord)
+    for k in range(-m, m + 1):
+        nodes = np.arange(-m, m + 1) - k
+        for deg in range(nord + 1):
+            got = float(np.dot(t[k + m], nodes.astype(float) ** deg))
+            want = 1.0 if deg == 1 else 0.0
+            scale = max(1.0, float(np.abs(t[k + m]).dot(np.abs(nodes.astype(float)) ** deg)))
+            assert abs(got - want) <= 1e-13 * scale, (nord, k, deg, got)
+
+
+def test_poisson_oracle_matches_reference_routines(golden_dir):
+    """div_u_i (du, cc) and nabla_u of the oracle against div_u_i_ref,
+    div_u_i_opt and nabla_u_ref run on the same fields (tests/golden/
+    poisson.npz).  With the Lagrange table the results agree everywhere; with
+    the reference's own nord-8 table they differ only at the first point of
+    each non-periodic line (the row that carries the typo)."""
+    g = np.load(os.path.join(golden_dir, 'poisson.npz'))
+    h = g['h']
+    for case in g['cases']:
+        nord, geo, tab = str(case).split('_')
+        nord, geo = int(nord), int(geo)
+        key = 'big' if nord == 16 else 'small'
+        n = [int(v) for v in g['n_' + key]]
+        u = g['u_' + key]
+        uf = u.reshape(-1, order='F').copy()
+        np_ = int(np.prod(n))
+        du, cc = np.zeros(np_), np.zeros(np_)
+        O.div_u_i(geo, n, uf, du, nord, h, cc)
+        gr = np.zeros(3 * np_)
+        O.nabla_u(geo, n, u[..., 0].reshape(-1, order='F').copy(), gr, nord, h)
+        ref_du = g['div_' + str(case)].reshape(-1, order='F')
+        ref_du2 = g['divopt_' + str(case)].reshape(-1, order='F')
+        ref_cc = g['cc_' + str(case)].reshape(-1, order='F')
+        ref_gr = g['nabla_' + str(case)].reshape(-1, order='F')
+        if nord == 8 and tab == 'ref':
+            per = [geo != 1, geo == 2, geo != 1]
+            idx = np.indices(n[::-1])[::-1]  # i0, i1, i2 of every point, Fortran order
+            first = np.zeros(n[::-1], dtype=bool)
+            for k in range(3):
+                if not per[k]:
+                    first |= idx[k] == 0
+            mask = ~first.reshape(-1)
+            assert rel(du[mask], ref_du[mask]) < 1e-14
+            assert not np.allclose(du[~mask], ref_du[~mask])  # the typo shows only there
+            continue
+        assert rel(du, ref_du) < 1e-14, case                  # div_u_i_ref: same sums
+        assert rel(du, ref_du2) < 1e-13, case
+        assert rel(cc, ref_cc) < 1e-13, case
+        assert rel(gr, ref_gr.reshape(-1)) < 1e-13, case
+
+
+def test_poisson_broker_and_plane_waves():
+    """The 1D operator (Poisson_broker) is one component of nabla_u; on a
+    periodic line it maps the plane wave e^{i theta j} to
+    (sum_j c_j e^{i theta j}) times itself; a non-periodic line
+    differentiates a polynomial of degree nord exactly, borders included."""
+    rng = np.random.default_rng(77)
+    n = [20, 18, 19]
+    u = rng.uniform(-1, 1, int(np.prod(n)))
+    h = [0.3, 0.4, 0.5]
+    for nord in O.POISSON_ORDERS:
+        gr = np.zeros(3 * u.size)
+        O.nabla_u(3, n, u, gr, nord, h)
+        for k, bc in ((0, 0), (1, -2), (2, 0)):
+            d = np.zeros(u.size)
+            O.poisson_1d(nord, k, n, bc, u, d, a=1.0 / h[k])
+            assert rel(d, gr[k * u.size:(k + 1) * u.size]) < 1e-14
+    for nord in O.POISSON_ORDERS:
+        m = nord // 2
+        c = O.poisson_taps(nord)[m]
+        L = 24
+        for q in (1, 3, 5):
+            th = 2 * np.pi * q / L
+            x = np.cos(th * np.arange(L))
+            nn = [L, nord + 1, nord + 1]
+            xs = np.tile(x, nn[1] * nn[2])
+            d = np.zeros(xs.size)
+            O.poisson_1d(nord, 0, nn, 0, xs, d)
+            lam = np.sum(c * np.sin(th * np.arange(-m, m + 1)))  # D cos = -lam sin for odd weights
+            assert np.abs(d[:L] + lam * np.sin(th * np.arange(L))).max() < 1e-13
+        xs = np.arange(nord + 3, dtype=float) / (nord + 3)
+        nn = [nord + 3, nord + 1, nord + 1]
+        p = np.tile(xs ** nord, nn[1] * nn[2])
+        d = np.zeros(p.size)
+        O.poisson_1d(nord, 0, nn, -2, p, d, a=float(nord + 3))
+        want = nord * xs ** (nord - 1)
+        assert np.abs(d[:nord + 3] - want).max() < 1e-9 * max(1.0, np.abs(want).max())
