@@ -215,6 +215,36 @@ def run_hamiltonian(torch, lc, dtype, steps, warmup, n=64):
             'fine_points_per_s_G': round(nf / (ms * 1e-3) / 1e9, 3)}
 
 
+def run_poisson(torch, lc, dtype, steps, warmup, n=256, nord=16, geocode=3):
+    """div_u_i with cc on an n^3 vector field, order 16, geocode 3 (x, z
+    periodic, y free): the configuration of BigDFT/Poisson_test2.rb."""
+    nn = [n] * 3
+    npts = n ** 3
+    g = torch.Generator(device='cuda').manual_seed(9)
+    u = torch.rand(3 * npts, dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+    du = torch.empty(npts, dtype=dtype, device='cuda')
+    cc = torch.empty(npts, dtype=dtype, device='cuda')
+    h = [0.1, 0.1, 0.1]
+    st = torch.cuda.current_stream()
+    for _ in range(warmup):
+        lc.div_u_i(geocode, nn, u, du, nord, h, cc, stream=st)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with quiet_host():
+        e0.record(st)
+        for _ in range(steps):
+            lc.div_u_i(geocode, nn, u, du, nord, h, cc, stream=st)
+        e1.record(st)
+        torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    sz = 4 if dtype == torch.float32 else 8
+    byts = sz * 5 * npts  # u (3 components) in, du and cc out
+    return {'n': nn, 'nord': nord, 'geocode': geocode, 'ms': round(ms, 5),
+            'GB_s': round(byts / (ms * 1e-3) / 1e9, 1),
+            'frac_hbm': round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), 'algorithmic_bytes': byts,
+            'Gpts_s': round(npts / (ms * 1e-3) / 1e9, 3)}
+
+
 def batch_orbitals(count=512, seed=5):
     """Orbital sizes ~U{24..96} per dim and BC ~U{periodic, grow, shrink}."""
     import numpy as np
@@ -448,6 +478,8 @@ def main():
         out['dwt3d']['fp32'] = run_dwt(torch, lc, torch.float32, args.steps, args.warmup)
         out['hamiltonian'] = run_hamiltonian(torch, lc, torch.float64, args.steps, args.warmup)
         out['hamiltonian']['fp32'] = run_hamiltonian(torch, lc, torch.float32, args.steps, args.warmup)
+        out['poisson_div'] = run_poisson(torch, lc, torch.float64, args.steps, args.warmup)
+        out['poisson_div']['fp32'] = run_poisson(torch, lc, torch.float32, args.steps, args.warmup)
         out['batch'] = run_batch(torch, lc, torch.float64, reps=3)
         out['batch']['fp32'] = run_batch(torch, lc, torch.float32, reps=3)
     if not args.no_oracle:
