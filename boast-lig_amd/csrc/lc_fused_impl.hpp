@@ -276,7 +276,7 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
   X(16, 32, 16, 4, 4, 2, 4, 0, 0) X(17, 32, 16, 4, 4, 2, 2, 0, 0) X(18, 64, 8, 4, 4, 2, 4, 0, 0)       \
   X(19, 32, 16, 8, 8, 4, 2, 0, 0) X(20, 32, 16, 4, 4, 2, 4, 8, 16) X(21, 32, 16, 8, 8, 4, 2, 8, 32)    \
   X(22, 16, 16, 4, 4, 2, 4, 8, 16) X(23, 16, 16, 4, 4, 2, 2, 0, 16) X(24, 32, 8, 4, 4, 2, 3, 0, 16)    \
-  X(25, 32, 8, 4, 4, 2, 2, 0, 16)
+  X(25, 32, 8, 4, 4, 2, 2, 0, 16) X(26, 16, 16, 4, 4, 2, 2, 0, 64) X(27, 16, 16, 4, 4, 2, 4, 8, 64)
 // two-plane separable form (sep only): tile, QX / QY outputs per task, QZ
 // planes per step, waves per EU, shortest z chunk (0: 4 W, ~2 blocks per CU;
 // else chunks down to ZC planes for ~4 blocks per CU), prefetch depth in steps
@@ -285,7 +285,7 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
   X(19, 32, 8, 8, 2, 2, 3, 0, 1) X(20, 64, 8, 8, 2, 2, 2, 0, 1) X(21, 32, 32, 4, 4, 2, 2, 0, 1)          \
   X(22, 32, 16, 4, 4, 2, 2, 32, 1) X(23, 32, 16, 4, 4, 2, 3, 32, 1) X(24, 32, 16, 4, 4, 2, 2, 48, 1)     \
   X(25, 32, 16, 4, 4, 2, 2, 0, 2) X(26, 32, 16, 4, 2, 2, 4, 0, 2) X(27, 32, 16, 4, 4, 2, 2, 48, 2)
-constexpr int kFusedVariantCount = 26;
+constexpr int kFusedVariantCount = 28;
 constexpr int kSepVariantCount = 28;
 
 template <typename R, class F>
