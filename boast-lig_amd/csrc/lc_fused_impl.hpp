@@ -284,9 +284,10 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
   X(16, 32, 16, 4, 2, 2, 2, 0, 1) X(17, 32, 16, 4, 4, 2, 2, 0, 1) X(18, 32, 16, 4, 2, 2, 4, 0, 1)        \
   X(19, 32, 8, 8, 2, 2, 3, 0, 1) X(20, 64, 8, 8, 2, 2, 2, 0, 1) X(21, 32, 32, 4, 4, 2, 2, 0, 1)          \
   X(22, 32, 16, 4, 4, 2, 2, 32, 1) X(23, 32, 16, 4, 4, 2, 3, 32, 1) X(24, 32, 16, 4, 4, 2, 2, 48, 1)     \
-  X(25, 32, 16, 4, 4, 2, 2, 0, 2) X(26, 32, 16, 4, 2, 2, 4, 0, 2) X(27, 32, 16, 4, 4, 2, 2, 48, 2)
+  X(25, 32, 16, 4, 4, 2, 2, 0, 2) X(26, 32, 16, 4, 2, 2, 4, 0, 2) X(27, 32, 16, 4, 4, 2, 2, 48, 2)          \
+  X(28, 32, 16, 4, 4, 4, 1, 0, 1) X(29, 32, 16, 4, 4, 4, 2, 0, 1) X(30, 32, 16, 4, 4, 4, 2, 48, 1)
 constexpr int kFusedVariantCount = 28;
-constexpr int kSepVariantCount = 28;
+constexpr int kSepVariantCount = 31;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,

@@ -17,7 +17,7 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 
 
 def test_variant_counts(lc):
-    assert lc.variant_count(lc.FORM_SEP) == 28
+    assert lc.variant_count(lc.FORM_SEP) == 31
     assert lc.variant_count(lc.FORM_SUM) == 28
     assert lc.variant_count(lc.FORM_DWT) == 4
     assert lc.variant_count(lc.FORM_IDWT) == 4
@@ -27,7 +27,7 @@ def test_default_kernel_names(lc):
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<double, lc::FD2, 16, 16, 4, 4, 2, 2, 0>'
     assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<float, lc::FD2, 32, 16, 8, 8, 4, 2, 8>'
     assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
-        'sep3d2_fused<float, lc::FMD, 32, 16, 4, 4, 2, 2, 2>'
+        'sep3d2_fused<float, lc::FMD, 32, 16, 4, 4, 4, 1, 1>'
     assert lc.kernel_name(8, lc.FORM_IDWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).startswith(
         'sepw3d_fused<double, lc::Idwt<')
 
