@@ -20,7 +20,11 @@
 
 namespace lc {
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ>
+// PDA: prefetch depth in steps (1 or 2), plus 4 when P shares A's LDS (A is
+// dead once the x-tasks have read it; one more barrier per step, before the
+// next step writes A, buys 2/3 of the LDS and so two FP64 blocks per CU at
+// four planes per step)
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int PDA = 1>
 struct Sep2Geo {
   static constexpr int W = F::N;
   static constexpr int NT = TX * TY;
@@ -31,22 +35,27 @@ struct Sep2Geo {
   static constexpr int NAT = QZ * NAP;                // per lane per step
   static constexpr int NXT = QZ * TH * (TX / QX);
   static constexpr int NYT = QZ * TX * (TY / QY);
+  static constexpr bool ALIAS = (PDA & 4) != 0;
+  static constexpr int PD = PDA & 3;
   static constexpr size_t lds_bytes() {
     return lds_align(sizeof(R) * QZ * TH * TWP) + lds_align(sizeof(R) * QZ * TH * BW) +
-           lds_align(sizeof(R) * QZ * TY * BW);
+           (ALIAS ? 0 : lds_align(sizeof(R) * QZ * TY * BW));
   }
+  static_assert(PD == 1 || PD == 2, "prefetch depth");
+  static_assert(TY * BW <= TH * TWP, "P fits in A");
   static_assert(TX % QX == 0 && TY % QY == 0 && NT % 64 == 0, "tile shape");
 };
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int PD>
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int PDA>
 __device__ __forceinline__ void sep3d2_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
                                             unsigned char* lds) {
-  using G = Sep2Geo<R, F, TX, TY, QX, QY, QZ>;
+  using G = Sep2Geo<R, F, TX, TY, QX, QY, QZ, PDA>;
+  constexpr int PD = G::PD;
   constexpr int W = G::W, NT = G::NT, TWD = G::TWD, TH = G::TH, TWP = G::TWP, BW = G::BW;
   constexpr int NA = G::NA, NAP = G::NAP, NAT = G::NAT;
   R* A = reinterpret_cast<R*>(lds);                                          // [QZ][TH][TWP]
   R* B = reinterpret_cast<R*>(lds + lds_align(sizeof(R) * QZ * TH * TWP));  // [QZ][TH][BW]
-  R* P = B + lds_align(sizeof(R) * QZ * TH * BW) / sizeof(R);               // [QZ][TY][BW]
+  R* P = G::ALIAS ? A : B + lds_align(sizeof(R) * QZ * TH * BW) / sizeof(R);  // [QZ][TY][BW]
 
   const int t = threadIdx.x;
   const R* __restrict__ xa = p.x + arr * p.nti;
@@ -158,6 +167,7 @@ __device__ __forceinline__ void sep3d2_body(const FusedArgs<R>& p, int bx, int b
     }
 #pragma unroll
     for (int s = 0; s < W - 1; ++s) win[s] = win[s + QZ];
+    if (G::ALIAS) __syncthreads();  // every lane has read P before the next step writes A
   };
 
   R areg[PD][NAT];  // prefetch depth PD: buffer u serves steps u, u + PD, ...

@@ -157,7 +157,7 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     // defaults: the fastest variants at 256^3 on MI355X (tuner sweep); the
     // two-plane forms (16 and up) take planes of at most kPlaneMax bytes
     const bool fit = plane_fits(b, prec);
-    if (form == FORM_SEP) return !fit ? 12 : prec == 4 ? 28 : 17;
+    if (form == FORM_SEP) return !fit ? 12 : prec == 4 ? 28 : 33;
     if (!fit) return prec == 4 ? 7 : 8;
     return prec == 4 ? 21 : 26;
   }
@@ -201,7 +201,7 @@ hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R 
                            const R* pot, R* rho) {
   int v = select_variant(sizeof(R), FORM_SEP, id, b);
   if (pot && v >= 5 && v <= 7)  // the wave-specialised forms have no potential step
-    v = plane_fits(b, sizeof(R)) ? (sizeof(R) == 4 ? 28 : 17) : 12;
+    v = plane_fits(b, sizeof(R)) ? (sizeof(R) == 4 ? 28 : 33) : 12;
   return sep3d_fused_v<R>(v, id, b, narr, x, y, a, a_y, st, pot, rho);
 }
 

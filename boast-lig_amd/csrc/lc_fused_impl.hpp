@@ -197,7 +197,7 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
 template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int PD>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d2_fused(
     FusedArgs<R> p) {
-  __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ, PD>::lds_bytes()];
   int bx, by, zc, arr;
   block_coords(p.tiles_x, bx, by, zc, arr);
   sep3d2_body<R, F, TX, TY, QX, QY, QZ, PD>(p, bx, by, zc, arr, lds);
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const int local = lin - first[o];
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = local % tiles, zc = local / tiles;
-  __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  __shared__ __align__(16) unsigned char lds[Sep2Geo<R, F, TX, TY, QX, QY, QZ, PD>::lds_bytes()];
   sep3d2_body<R, F, TX, TY, QX, QY, QZ, PD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
 }
 
@@ -280,14 +280,17 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
 // two-plane separable form (sep only): tile, QX / QY outputs per task, QZ
 // planes per step, waves per EU, shortest z chunk (0: 4 W, ~2 blocks per CU;
 // else chunks down to ZC planes for ~4 blocks per CU), prefetch depth in steps
+// (+4: P shares A's LDS, see Sep2Geo)
 #define LC_SEP2_VARIANTS(X)                                                                        \
   X(16, 32, 16, 4, 2, 2, 2, 0, 1) X(17, 32, 16, 4, 4, 2, 2, 0, 1) X(18, 32, 16, 4, 2, 2, 4, 0, 1)        \
   X(19, 32, 8, 8, 2, 2, 3, 0, 1) X(20, 64, 8, 8, 2, 2, 2, 0, 1) X(21, 32, 32, 4, 4, 2, 2, 0, 1)          \
   X(22, 32, 16, 4, 4, 2, 2, 32, 1) X(23, 32, 16, 4, 4, 2, 3, 32, 1) X(24, 32, 16, 4, 4, 2, 2, 48, 1)     \
   X(25, 32, 16, 4, 4, 2, 2, 0, 2) X(26, 32, 16, 4, 2, 2, 4, 0, 2) X(27, 32, 16, 4, 4, 2, 2, 48, 2)          \
-  X(28, 32, 16, 4, 4, 4, 1, 0, 1) X(29, 32, 16, 4, 4, 4, 2, 0, 1) X(30, 32, 16, 4, 4, 4, 2, 48, 1)
+  X(28, 32, 16, 4, 4, 4, 1, 0, 1) X(29, 32, 16, 4, 4, 4, 2, 0, 1) X(30, 32, 16, 4, 4, 4, 2, 48, 1)          \
+  X(31, 32, 16, 4, 4, 4, 2, 0, 5) X(32, 32, 16, 4, 4, 4, 2, 32, 5) X(33, 32, 16, 4, 4, 2, 2, 0, 5)           \
+  X(34, 32, 16, 4, 4, 2, 2, 32, 5) X(35, 32, 16, 4, 2, 2, 4, 0, 5) X(36, 32, 16, 4, 4, 2, 3, 0, 6)
 constexpr int kFusedVariantCount = 28;
-constexpr int kSepVariantCount = 31;
+constexpr int kSepVariantCount = 37;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
