@@ -169,10 +169,13 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
     p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
   } else {
     // a z chunk only costs the window prefill (W - 1 column loads per lane):
-    // chunks down to ZC planes put ~8 blocks on every CU
-    const long want = 8L * num_cu();
-    long chunks = (want + tiles * std::max(narr, 1) - 1) / (tiles * std::max(narr, 1));
+    // chunks down to ZC planes put ~8 blocks on every CU, and down to 16
+    // planes when that would leave CUs without a block
+    const long per = tiles * std::max(narr, 1);
+    long chunks = (8L * num_cu() + per - 1) / per;
     chunks = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / ZC)));
+    if (chunks * per < num_cu())
+      chunks = std::max(chunks, std::min<long>((num_cu() + per - 1) / per, std::max(1, p.d2.nout / 16)));
     p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
     p.zchunk = (p.zchunk + QZ - 1) / QZ * QZ;
   }
@@ -235,9 +238,11 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
     p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
   } else {
     // a chunk recomputes W - 1 filtered planes: shorter chunks buy occupancy
-    const long want = 4L * num_cu();
-    long chunks = (want + tiles * std::max(narr, 1) - 1) / (tiles * std::max(narr, 1));
+    const long per = tiles * std::max(narr, 1);
+    long chunks = (4L * num_cu() + per - 1) / per;
     chunks = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / ZC)));
+    if (chunks * per < num_cu())
+      chunks = std::max(chunks, std::min<long>((num_cu() + per - 1) / per, std::max(1, p.d2.nout / (2 * F::N))));
     p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
   }
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
