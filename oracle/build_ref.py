@@ -62,7 +62,12 @@ def notebook_result(path, cell):
     raise RuntimeError('%s: cell %d has no execute_result' % (path, cell))
 
 
+FORCE = False  # --force: rebuild libraries that already exist
+
+
 def compile_so(sources, out, tmp):
+    if not FORCE and os.path.exists(out) and os.path.getmtime(out) >= os.path.getmtime(__file__):
+        return  # built from the same recipe: the reference checkout is read-only
     paths = []
     for name, text in sources:
         if text is None:
@@ -80,7 +85,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--ref', default='/root/reference')
     ap.add_argument('--out', default=os.path.join(os.path.dirname(os.path.abspath(__file__)), '_ref'))
+    ap.add_argument('--force', action='store_true', help='rebuild every library')
     args = ap.parse_args()
+    global FORCE
+    FORCE = args.force
     bd = os.path.join(args.ref, 'BigDFT')
     if not os.path.isdir(bd):
         print('build_ref: no reference checkout at %s; skipping' % args.ref)
