@@ -11,7 +11,9 @@
 // kernels are HBM / L2 bound: one lane per output point, x fastest across the
 // lanes of a wave (coalesced loads and stores); the taps of one output along
 // a strided dim are loads of the same lanes at a fixed offset, which the L2
-// serves for the neighbouring planes.  Each operator is one pass:
+// serves: each XCD works through its own slab of every plane in z order
+// (pblock), so the planes a z stencil needs are in its L2.  Each operator is
+// one pass:
 //   poisson_1d:          du = a D_idim u (+ a_y du)          (Poisson_broker)
 //   div_u_i:             du = sum_k D_k u_k / h_k, optional cc
 //   nabla_u:             du_k = D_k u / h_k
@@ -112,55 +114,79 @@ __device__ __forceinline__ R dk(const PGeom& g, const R* __restrict__ f, long p,
 
 constexpr int kPThreads = 256;
 
-template <typename R, int NORD>
-__global__ __launch_bounds__(kPThreads) void poisson_1d_kernel(PGeom g, int idim, const R* __restrict__ u,
-                                                               R* __restrict__ du, R a, R a_y) {
-  for (long p = (long)blockIdx.x * kPThreads + threadIdx.x; p < g.np; p += (long)gridDim.x * kPThreads) {
-    int i0, i1, i2;
-    pcoords(g, p, i0, i1, i2);
-    R t = dk<R, NORD>(g, u, p, idim, i0, i1, i2) * a;
-    if (a_y != R(0)) t = fmar(a_y, du[p], t);
-    du[p] = t;
-  }
+// Block -> (plane z, 256-point segment of the plane).  The dispatcher hands
+// consecutive block ids to the 8 XCDs in turn; XCD k serves the segments
+// [k S / 8, (k + 1) S / 8) of every plane, plane after plane, so the 2m + 1
+// planes a z stencil reads were loaded into that XCD's L2 by its own recent
+// blocks.  Blocks beyond their XCD's share exit at once.
+__device__ __forceinline__ bool pblock(const PGeom& g, int segs, long& p0, int& count) {
+  const int b = blockIdx.x, x = b & 7;
+  const long l = b >> 3;
+  const int lo = (int)((long)segs * x / 8), hi = (int)((long)segs * (x + 1) / 8);
+  const int c = hi - lo;
+  if (c <= 0 || l >= (long)c * g.n2) return false;
+  const long plane = (long)g.n0 * g.n1;
+  const int z = (int)(l / c), seg = lo + (int)(l % c);
+  p0 = (long)z * plane + (long)seg * kPThreads;
+  count = (int)std::min<long>(kPThreads, plane - (long)seg * kPThreads);
+  return true;
 }
 
 template <typename R, int NORD>
-__global__ __launch_bounds__(kPThreads) void div_u_i_kernel(PGeom g, const R* __restrict__ u, R* __restrict__ du,
-                                                            R* __restrict__ cc, R h0, R h1, R h2) {
+__global__ __launch_bounds__(kPThreads) void poisson_1d_kernel(PGeom g, int segs, int idim, const R* __restrict__ u,
+                                                               R* __restrict__ du, R a, R a_y) {
+  long p0;
+  int count;
+  if (!pblock(g, segs, p0, count) || (int)threadIdx.x >= count) return;
+  const long p = p0 + threadIdx.x;
+  int i0, i1, i2;
+  pcoords(g, p, i0, i1, i2);
+  R t = dk<R, NORD>(g, u, p, idim, i0, i1, i2) * a;
+  if (a_y != R(0)) t = fmar(a_y, du[p], t);
+  du[p] = t;
+}
+
+template <typename R, int NORD>
+__global__ __launch_bounds__(kPThreads) void div_u_i_kernel(PGeom g, int segs, const R* __restrict__ u,
+                                                            R* __restrict__ du, R* __restrict__ cc, R h0, R h1, R h2) {
+  long p0;
+  int count;
+  if (!pblock(g, segs, p0, count) || (int)threadIdx.x >= count) return;
+  const long p = p0 + threadIdx.x;
   const R* u0 = u;
   const R* u1 = u + g.np;
   const R* u2 = u + 2 * g.np;
-  for (long p = (long)blockIdx.x * kPThreads + threadIdx.x; p < g.np; p += (long)gridDim.x * kPThreads) {
-    int i0, i1, i2;
-    pcoords(g, p, i0, i1, i2);
-    const R dx = dk<R, NORD>(g, u0, p, 0, i0, i1, i2) / h0;
-    const R dy = dk<R, NORD>(g, u1, p, 1, i0, i1, i2) / h1;
-    const R dz = dk<R, NORD>(g, u2, p, 2, i0, i1, i2) / h2;
-    du[p] = (dx + dy) + dz;
-    if (cc) {  // uniform
-      const R x = u0[p], y = u1[p], z = u2[p];
-      const R xy = dk<R, NORD>(g, u1, p, 0, i0, i1, i2) / h0;
-      const R xz = dk<R, NORD>(g, u2, p, 0, i0, i1, i2) / h0;
-      const R yz = dk<R, NORD>(g, u2, p, 1, i0, i1, i2) / h1;
-      cc[p] = (((x * x) * dx + (y * y) * dy) + (z * z) * dz) + R(2) * ((x * y * xy + x * z * xz) + y * z * yz);
-    }
+  int i0, i1, i2;
+  pcoords(g, p, i0, i1, i2);
+  const R dx = dk<R, NORD>(g, u0, p, 0, i0, i1, i2) / h0;
+  const R dy = dk<R, NORD>(g, u1, p, 1, i0, i1, i2) / h1;
+  const R dz = dk<R, NORD>(g, u2, p, 2, i0, i1, i2) / h2;
+  du[p] = (dx + dy) + dz;
+  if (cc) {  // uniform
+    const R x = u0[p], y = u1[p], z = u2[p];
+    const R xy = dk<R, NORD>(g, u1, p, 0, i0, i1, i2) / h0;
+    const R xz = dk<R, NORD>(g, u2, p, 0, i0, i1, i2) / h0;
+    const R yz = dk<R, NORD>(g, u2, p, 1, i0, i1, i2) / h1;
+    cc[p] = (((x * x) * dx + (y * y) * dy) + (z * z) * dz) + R(2) * ((x * y * xy + x * z * xz) + y * z * yz);
   }
 }
 
 template <typename R, int NORD>
-__global__ __launch_bounds__(kPThreads) void nabla_u_kernel(PGeom g, const R* __restrict__ u, R* __restrict__ du,
-                                                            R* __restrict__ du2, R h0, R h1, R h2) {
-  for (long p = (long)blockIdx.x * kPThreads + threadIdx.x; p < g.np; p += (long)gridDim.x * kPThreads) {
-    int i0, i1, i2;
-    pcoords(g, p, i0, i1, i2);
-    const R dx = dk<R, NORD>(g, u, p, 0, i0, i1, i2) / h0;
-    const R dy = dk<R, NORD>(g, u, p, 1, i0, i1, i2) / h1;
-    const R dz = dk<R, NORD>(g, u, p, 2, i0, i1, i2) / h2;
-    du[p] = dx;
-    du[g.np + p] = dy;
-    du[2 * g.np + p] = dz;
-    if (du2) du2[p] = (dx * dx + dy * dy) + dz * dz;  // uniform branch
-  }
+__global__ __launch_bounds__(kPThreads) void nabla_u_kernel(PGeom g, int segs, const R* __restrict__ u,
+                                                            R* __restrict__ du, R* __restrict__ du2, R h0, R h1, R h2) {
+  long p0;
+  int count;
+  if (!pblock(g, segs, p0, count) || (int)threadIdx.x >= count) return;
+  const long p = p0 + threadIdx.x;
+  int i0, i1, i2;
+  pcoords(g, p, i0, i1, i2);
+  const R dx = dk<R, NORD>(g, u, p, 0, i0, i1, i2) / h0;
+  const R dy = dk<R, NORD>(g, u, p, 1, i0, i1, i2) / h1;
+  const R dz = dk<R, NORD>(g, u, p, 2, i0, i1, i2) / h2;
+  du[p] = dx;
+  du[g.np + p] = dy;
+  du[2 * g.np + p] = dz;
+  if (du2) du2[p] = (dx * dx + dy * dy) + dz * dz;  // uniform branch
 }
 
 PGeom pgeom(const int* n, int geocode, int bc_idim, int idim) {
@@ -177,10 +203,10 @@ PGeom pgeom(const int* n, int geocode, int bc_idim, int idim) {
   return g;
 }
 
-unsigned pgrid(long np) {
-  const long b = (np + kPThreads - 1) / kPThreads;
-  return (unsigned)std::min<long>(std::max<long>(b, 1), 64L * 1024);
-}
+int psegs(const PGeom& g) { return (int)(((long)g.n0 * g.n1 + kPThreads - 1) / kPThreads); }
+
+// 8 XCD shares of ceil(S / 8) segments times n2 planes
+unsigned pgrid(const PGeom& g) { return (unsigned)(8L * ((psegs(g) + 7) / 8) * g.n2); }
 
 }  // namespace
 
@@ -200,7 +226,7 @@ template <typename R>
 hipError_t poisson_1d(int nord, int idim, const int* n, int bc, const R* u, R* du, R a, R a_y, hipStream_t st) {
   const PGeom g = pgeom(n, 0, bc, idim);
   if (g.np == 0) return hipSuccess;
-  LC_POISSON_DISPATCH(nord, (poisson_1d_kernel<R, NO><<<pgrid(g.np), kPThreads, 0, st>>>(g, idim, u, du, a, a_y)))
+  LC_POISSON_DISPATCH(nord, (poisson_1d_kernel<R, NO><<<pgrid(g), kPThreads, 0, st>>>(g, psegs(g), idim, u, du, a, a_y)))
   return hipGetLastError();
 }
 
@@ -208,7 +234,7 @@ template <typename R>
 hipError_t div_u_i(int geocode, const int* n, const R* u, R* du, R* cc, int nord, const double* h, hipStream_t st) {
   const PGeom g = pgeom(n, geocode, 0, -1);
   if (g.np == 0) return hipSuccess;
-  LC_POISSON_DISPATCH(nord, (div_u_i_kernel<R, NO><<<pgrid(g.np), kPThreads, 0, st>>>(g, u, du, cc, (R)h[0],
+  LC_POISSON_DISPATCH(nord, (div_u_i_kernel<R, NO><<<pgrid(g), kPThreads, 0, st>>>(g, psegs(g), u, du, cc, (R)h[0],
                                                                                        (R)h[1], (R)h[2])))
   return hipGetLastError();
 }
@@ -217,7 +243,7 @@ template <typename R>
 hipError_t nabla_u(int geocode, const int* n, const R* u, R* du, R* du2, int nord, const double* h, hipStream_t st) {
   const PGeom g = pgeom(n, geocode, 0, -1);
   if (g.np == 0) return hipSuccess;
-  LC_POISSON_DISPATCH(nord, (nabla_u_kernel<R, NO><<<pgrid(g.np), kPThreads, 0, st>>>(g, u, du, du2, (R)h[0],
+  LC_POISSON_DISPATCH(nord, (nabla_u_kernel<R, NO><<<pgrid(g), kPThreads, 0, st>>>(g, psegs(g), u, du, du2, (R)h[0],
                                                                                        (R)h[1], (R)h[2])))
   return hipGetLastError();
 }
