@@ -28,17 +28,18 @@
 
 namespace lc {
 
-template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
-__global__ __launch_bounds__(2 * TX * TY) void sepw3d_fused(FusedArgs<R> p) {
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD, int WPE>
+__global__ __launch_bounds__(2 * TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sepw3d_fused(
+    FusedArgs<R> p) {
   __shared__ __align__(16) unsigned char lds[sepw3d_lds_bytes<R, Op, TX, TY, QX, BZ, AD>()];
   int bx, by, zc, arr;
   block_coords(p.tiles_x, bx, by, zc, arr);
   sepw3d_body<R, Op, TX, TY, QX, BZ, AD>(p, bx, by, zc, arr, lds);
 }
 
-template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
-__global__ __launch_bounds__(2 * TX * TY) void sepw3d_batch(const FusedArgs<R>* __restrict__ tab,
-                                                             const int* __restrict__ first, int count) {
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD, int WPE>
+__global__ __launch_bounds__(2 * TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sepw3d_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
   const int lin = xcd_linear();
   const int o = batch_item(first, count, lin);
   const FusedArgs<R> p = tab[o];
@@ -51,11 +52,12 @@ __global__ __launch_bounds__(2 * TX * TY) void sepw3d_batch(const FusedArgs<R>* 
 
 // ====================================================================== host
 // Compiled variants: tile TX x TY steps, QX outputs per x-task, BZ steps per
-// window slide, AD prefetch depth (2 TX TY lanes per block).
-#define LC_WAV_VARIANTS(X) \
-  X(0, 16, 8, 4, 2, 2) X(1, 8, 8, 4, 2, 2) X(2, 32, 4, 4, 2, 2) X(3, 16, 8, 4, 1, 1)
+// window slide, AD prefetch depth (2 TX TY lanes per block), waves per EU.
+#define LC_WAV_VARIANTS(X)                                                                  \
+  X(0, 16, 8, 4, 2, 2, 1) X(1, 8, 8, 4, 2, 2, 1) X(2, 32, 4, 4, 2, 2, 1) X(3, 16, 8, 4, 1, 1, 1) \
+  X(4, 16, 8, 4, 1, 1, 2)
 
-int wav_variant_count() { return 4; }
+int wav_variant_count() { return 5; }
 
 std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
   (void)id;
@@ -65,10 +67,10 @@ std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
   char buf[200];
   buf[0] = 0;
   switch (v) {
-#define X(I, TX, TY, QX, BZ, AD)                                                                   \
+#define X(I, TX, TY, QX, BZ, AD, WPE)                                                                   \
   case I:                                                                                         \
-    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d>", batch ? "sepw3d_batch" : "sepw3d_fused", \
-             r, op, TX, TY, QX, BZ, AD);                                                          \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>", batch ? "sepw3d_batch" : "sepw3d_fused", \
+             r, op, TX, TY, QX, BZ, AD, WPE);                                                        \
     break;
     LC_WAV_VARIANTS(X)
 #undef X
@@ -93,7 +95,7 @@ static FusedArgs<R> wav_args(const Box3& b, int narr, const R* x, R* y, R a, R a
   return p;
 }
 
-template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD, int WPE>
 static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
   if (!wav_plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, BZ, AD>(b, narr, x, y, a, a_y);
@@ -104,7 +106,7 @@ static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, 
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
-  sepw3d_fused<R, Op, TX, TY, QX, BZ, AD><<<grid, 2 * TX * TY, 0, st>>>(p);
+  sepw3d_fused<R, Op, TX, TY, QX, BZ, AD, WPE><<<grid, 2 * TX * TY, 0, st>>>(p);
   return hipGetLastError();
 }
 
@@ -112,8 +114,8 @@ template <typename R, class Op>
 static hipError_t sepw3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
                                  hipStream_t st) {
   switch (v) {
-#define X(I, TX, TY, QX, BZ, AD) \
-  case I: return launch_sepw3d<R, Op, TX, TY, QX, BZ, AD>(b, narr, x, y, a, a_y, st);
+#define X(I, TX, TY, QX, BZ, AD, WPE) \
+  case I: return launch_sepw3d<R, Op, TX, TY, QX, BZ, AD, WPE>(b, narr, x, y, a, a_y, st);
     LC_WAV_VARIANTS(X)
 #undef X
   }
@@ -142,7 +144,7 @@ hipError_t sepw3d(int kind, int id, const Box3& b, int narr, const R* x, R* y, R
 }
 
 // ---- batches of independent 3D sym8 wavelet transforms
-template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD>
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD, int WPE>
 static hipError_t wav_stage_build(const BatchItem<R>* it, int count, BatchStage* s) {
   std::vector<FusedArgs<R>> tab(count);
   std::vector<int> first(count + 1, 0);
@@ -173,8 +175,8 @@ template <typename R, class Op>
 static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, BatchStage* s, hipStream_t st) {
   if (build) {
     switch (s->variant) {
-#define X(I, TX, TY, QX, BZ, AD) \
-  case I: return wav_stage_build<R, Op, TX, TY, QX, BZ, AD>(it, count, s);
+#define X(I, TX, TY, QX, BZ, AD, WPE) \
+  case I: return wav_stage_build<R, Op, TX, TY, QX, BZ, AD, WPE>(it, count, s);
       LC_WAV_VARIANTS(X)
 #undef X
     }
@@ -183,9 +185,9 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
   if (s->blocks == 0) return hipSuccess;
   const FusedArgs<R>* tab = static_cast<const FusedArgs<R>*>(s->tab);
   switch (s->variant) {
-#define X(I, TX, TY, QX, BZ, AD)                                                                     \
+#define X(I, TX, TY, QX, BZ, AD, WPE)                                                                     \
   case I:                                                                                           \
-    sepw3d_batch<R, Op, TX, TY, QX, BZ, AD><<<s->blocks, 2 * TX * TY, 0, st>>>(tab, s->first, s->count); \
+    sepw3d_batch<R, Op, TX, TY, QX, BZ, AD, WPE><<<s->blocks, 2 * TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
     LC_WAV_VARIANTS(X)
 #undef X

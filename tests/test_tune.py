@@ -19,8 +19,8 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 def test_variant_counts(lc):
     assert lc.variant_count(lc.FORM_SEP) == 37
     assert lc.variant_count(lc.FORM_SUM) == 28
-    assert lc.variant_count(lc.FORM_DWT) == 4
-    assert lc.variant_count(lc.FORM_IDWT) == 4
+    assert lc.variant_count(lc.FORM_DWT) == 5
+    assert lc.variant_count(lc.FORM_IDWT) == 5
 
 
 def test_default_kernel_names(lc):
@@ -60,7 +60,7 @@ def test_json_cache_round_trip(lc, tmp_path, monkeypatch):
                  '"nx": [8, 8, 8], "ny": [8, 8, 8], "variant": 42, "ms": 0.1}\n]}\n')
     lc.tune_load(str(p))
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_fused<double, lc::FD2, 64, 8, 2, 4, 4, 2, 1>'
-    assert ', 32, 4, 4, 2, 2>' in lc.kernel_name(8, lc.FORM_DWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3)
+    assert ', 32, 4, 4, 2, 2, 1>' in lc.kernel_name(8, lc.FORM_DWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3)
     # the entry with an unknown variant is ignored; other precisions are separate
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, [8] * 3, 0, [8] * 3, [8] * 3).startswith('sum3d_ws<')
     assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256).startswith('sum3d_ws<float')
