@@ -130,6 +130,13 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
     break;
       LC_SEP2_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC)                                                              \
+  case I:                                                                                           \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>", batch ? "sepp_batch" : "sepp_fused", \
+             r, filter_tag(id), TX, TY, QX, QY, PD, WPE);                                            \
+    break;
+      LC_SEPP_VARIANTS(X)
+#undef X
     }
   }
   switch (v) {
@@ -157,7 +164,7 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     // defaults: the fastest variants at 256^3 on MI355X (tuner sweep); the
     // two-plane forms (16 and up) take planes of at most kPlaneMax bytes
     const bool fit = plane_fits(b, prec);
-    if (form == FORM_SEP) return !fit ? 12 : prec == 4 ? 28 : 33;
+    if (form == FORM_SEP) return !fit ? 12 : 37;
     if (!fit) return prec == 4 ? 7 : 8;
     return prec == 4 ? 21 : 26;
   }

@@ -14,6 +14,7 @@
 #include "lc_body_conv.hpp"
 #include "lc_body_sum2.hpp"
 #include "lc_body_sep2.hpp"
+#include "lc_body_sepp.hpp"
 
 namespace lc {
 
@@ -219,11 +220,12 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   sep3d2_body<R, F, TX, TY, QX, QY, QZ, PD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
 }
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int ZC, int PD>
-hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
-                         const R* pot, R* rho) {
-  if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
-  FusedArgs<R> p{};
+// arguments and z chunk of the separable forms that read planes through
+// buffer records (two-plane and pipelined); returns the number of z chunks
+template <typename R, class F, int TX, int TY, int ZC>
+int sep_plane_args(const Box3& b, int narr, const R* x, R* y, R a, R a_y, const R* pot, R* rho,
+                   FusedArgs<R>& p) {
+  p = FusedArgs<R>{};
   p.x = x; p.y = y;
   p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
   p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
@@ -245,14 +247,76 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
       chunks = std::max(chunks, std::min<long>((num_cu() + per - 1) / per, std::max(1, p.d2.nout / (2 * F::N))));
     p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
   }
-  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   p.a0 = a; p.a_y = a_y;
   p.flags = (a != R(1) ? F_A : 0) | (a_y != R(0) ? F_AY : 0) | (pot ? F_POT : 0);
   p.pot = pot; p.rho = rho;
+  return (p.d2.nout + p.zchunk - 1) / p.zchunk;
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int ZC, int PD>
+hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                         const R* pot, R* rho) {
+  if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
+  FusedArgs<R> p;
+  const int zch = sep_plane_args<R, F, TX, TY, ZC>(b, narr, x, y, a, a_y, pot, rho, p);
   if (narr == 0) return hipSuccess;
-  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  dim3 grid((unsigned)(p.tiles_x * p.tiles_y), (unsigned)zch, (unsigned)narr);
   sep3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE, PD><<<grid, TX * TY, 0, st>>>(p);
   return hipGetLastError();
+}
+
+// ---- pipelined separable form (lc_body_sepp.hpp)
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sepp_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[SeppGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sepp_body<R, F, TX, TY, QX, QY, PD>(p, bx, by, zc, arr, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sepp_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[SeppGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
+  sepp_body<R, F, TX, TY, QX, QY, PD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+}
+
+// the pipelined form is built for the 16-tap filters (MagicFilter and its
+// transpose) only; other filters report an invalid variant
+template <class F>
+constexpr bool sepp_filter() { return F::N == 16; }
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE, int ZC>
+hipError_t launch_sepp(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                       const R* pot, R* rho) {
+  if constexpr (!sepp_filter<F>()) {
+    return hipErrorInvalidValue;
+  } else {
+    if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
+    FusedArgs<R> p;
+    const int zch = sep_plane_args<R, F, TX, TY, ZC>(b, narr, x, y, a, a_y, pot, rho, p);
+    if (narr == 0) return hipSuccess;
+    dim3 grid((unsigned)(p.tiles_x * p.tiles_y), (unsigned)zch, (unsigned)narr);
+    sepp_fused<R, F, TX, TY, QX, QY, PD, WPE><<<grid, TX * TY, 0, st>>>(p);
+    return hipGetLastError();
+  }
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipStream_t st) {
+  if constexpr (!sepp_filter<F>()) {
+    return hipErrorInvalidValue;
+  } else {
+    sepp_batch<R, F, TX, TY, QX, QY, PD, WPE><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count);
+    return hipSuccess;
+  }
 }
 
 // ====================================================================== variants
@@ -294,8 +358,14 @@ hipError_t launch_sep3d2(const Box3& b, int narr, const R* x, R* y, R a, R a_y, 
   X(28, 32, 16, 4, 4, 4, 1, 0, 1) X(29, 32, 16, 4, 4, 4, 2, 0, 1) X(30, 32, 16, 4, 4, 4, 2, 48, 1)          \
   X(31, 32, 16, 4, 4, 4, 2, 0, 5) X(32, 32, 16, 4, 4, 4, 2, 32, 5) X(33, 32, 16, 4, 4, 2, 2, 0, 5)           \
   X(34, 32, 16, 4, 4, 2, 2, 32, 5) X(35, 32, 16, 4, 2, 2, 4, 0, 5) X(36, 32, 16, 4, 4, 2, 3, 0, 6)
+// pipelined separable form (sep only, 16-tap filters): tile, QX / QY outputs
+// per task, prefetch depth in planes, waves per EU, shortest z chunk as above
+#define LC_SEPP_VARIANTS(X)                                                                       \
+  X(37, 32, 16, 4, 2, 2, 4, 0) X(38, 32, 16, 4, 2, 2, 6, 0) X(39, 32, 16, 4, 4, 2, 4, 0)            \
+  X(40, 32, 16, 4, 2, 1, 6, 0) X(41, 32, 16, 4, 2, 4, 4, 0) X(42, 32, 16, 8, 2, 2, 4, 0)            \
+  X(43, 32, 32, 4, 2, 2, 4, 0) X(44, 32, 16, 4, 2, 2, 4, 32)
 constexpr int kFusedVariantCount = 28;
-constexpr int kSepVariantCount = 37;
+constexpr int kSepVariantCount = 45;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
@@ -333,6 +403,10 @@ hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, 
 #define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PD) \
   case I: return launch_sep3d2<R, F, TX, TY, QX, QY, QZ, WPE, ZC, PD>(b, narr, x, y, a, a_y, st, pot, rho);
     LC_SEP2_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC) \
+  case I: return launch_sepp<R, F, TX, TY, QX, QY, PD, WPE, ZC>(b, narr, x, y, a, a_y, st, pot, rho);
+    LC_SEPP_VARIANTS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -427,6 +501,12 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
   case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
         LC_SEP2_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC)                                                                   \
+  case I:                                                                                                  \
+    return sepp_filter<F>() && items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) \
+                                                    : hipErrorInvalidValue;
+        LC_SEPP_VARIANTS(X)
+#undef X
       }
     }
     return hipErrorInvalidValue;
@@ -465,6 +545,14 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     sep3d2_batch<R, F, TX, TY, QX, QY, QZ, WPE, PD><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
       LC_SEP2_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC)                                       \
+  case I: {                                                                       \
+    const hipError_t eb = launch_sepp_batch<R, F, TX, TY, QX, QY, PD, WPE>(tab, s, st); \
+    if (eb != hipSuccess) return eb;                                              \
+    break;                                                                        \
+  }
+      LC_SEPP_VARIANTS(X)
 #undef X
       default: return hipErrorInvalidValue;
     }

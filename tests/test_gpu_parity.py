@@ -414,11 +414,12 @@ def test_local_hamiltonian_is_symmetric(lc, gpu):
     assert abs(float(torch.dot(tpsi, hp)) - e) < 1e-11 * abs(e)
 
 
-@pytest.mark.parametrize('v', [0, 6, 12, 17, 22, 25, 28, 33])
+@pytest.mark.parametrize('v', [0, 6, 12, 17, 22, 25, 28, 33, 37, 40])
 def test_local_hamiltonian_potential_step_in_every_form(lc, gpu, monkeypatch, v):
     """The potential / density step rides in the magic filter's store: the
-    single-plane (0, 12) and multi-plane (17, 22, 25, 28, 33) separable kernels take
-    it; a wave-specialised choice (6) falls back to the two-plane default."""
+    single-plane (0, 12), multi-plane (17, 22, 25, 28, 33) and pipelined (37, 40)
+    separable kernels take it; a wave-specialised choice (6) falls back to the
+    default."""
     torch = gpu
     monkeypatch.setenv('LIBCONV_SEP3D_VARIANT', str(v))
     rng = np.random.default_rng(950 + v)

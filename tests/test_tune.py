@@ -17,7 +17,7 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 
 
 def test_variant_counts(lc):
-    assert lc.variant_count(lc.FORM_SEP) == 37
+    assert lc.variant_count(lc.FORM_SEP) == 45
     assert lc.variant_count(lc.FORM_SUM) == 28
     assert lc.variant_count(lc.FORM_DWT) == 5
     assert lc.variant_count(lc.FORM_IDWT) == 5
@@ -27,18 +27,21 @@ def test_default_kernel_names(lc):
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<double, lc::FD2, 16, 16, 4, 4, 2, 2, 0>'
     assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<float, lc::FD2, 32, 16, 8, 8, 4, 2, 8>'
     assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
-        'sep3d2_fused<float, lc::FMD, 32, 16, 4, 4, 4, 1, 1>'
+        'sepp_fused<float, lc::FMD, 32, 16, 4, 2, 2, 4>'
+    assert lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
+        'sepp_fused<double, lc::FMD, 32, 16, 4, 2, 2, 4>'
     assert lc.kernel_name(8, lc.FORM_IDWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).startswith(
         'sepw3d_fused<double, lc::Idwt<')
 
 
 def test_planes_beyond_buffer_records_use_pointer_kernels(lc):
-    # the two-plane kernels read planes through 32-bit buffer records; a plane
-    # of more than 2^31 - 1 bytes selects the single-plane kernels instead
+    # the two-plane and pipelined kernels read planes through 32-bit buffer
+    # records; a plane of more than 2^31 - 1 bytes selects the single-plane
+    # kernels instead
     big = dict(n=[16400, 16400, 8], bc=0, nx=[16400, 16400, 8], ny=[16400, 16400, 8])
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **big).startswith('sum3d_fused<double')
     assert lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, **big).startswith('sep3d_fused<double')
-    assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, **big).startswith('sep3d2_fused<float')
+    assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, **big).startswith('sepp_fused<float')
 
 
 def test_environment_selects_variant(lc, monkeypatch):
