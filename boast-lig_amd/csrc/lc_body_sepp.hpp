@@ -117,10 +117,8 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
   const long syo = (long)p.d0.ny * p.d1.ny;
   long zo = (long)(zb - p.d2.oy) * syo;
   // source plane of the load issued in the next step (plane number PD + 1
-  // first), advanced one plane per step without a division: fsrc's rules
-  const bool zper = p.d2.bc == PER;
-  const int zlim = p.d2.bc == SHRINK ? p.d2.nx : p.d2.n;
-  int zraw = zper ? pmod(pfirst + PD + 1, p.d2.n) : pfirst + PD + 1 - (p.d2.bc == SHRINK ? p.d2.ox : 0);
+  // first), advanced one plane per step
+  ZCursor zc_ld(p.d2, pfirst + PD + 1);
 
   R ring[W];
 #pragma unroll
@@ -147,11 +145,10 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
         R* bq = buf[(u + 1) % PD];
 #pragma unroll
         for (int j = 0; j < NAP; ++j) a1[aslot[j]] = bq[j];
-        const int s2 = (unsigned)zraw < (unsigned)zlim ? zraw : -1;
-        const __amdgpu_buffer_rsrc_t rs = plane_rsrc(xa, sxy, s2);
+        const __amdgpu_buffer_rsrc_t rs = plane_rsrc(xa, sxy, zc_ld.src());
 #pragma unroll
         for (int j = 0; j < NAP; ++j) bq[j] = plane_ld<R>(rs, aoff[j]);
-        zraw = (zper && zraw + 1 == p.d2.n) ? 0 : zraw + 1;
+        zc_ld.next();
       }
       // ---- x-task on plane s
       if (xo >= 0) {

@@ -72,7 +72,6 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
   const int coff = (s0c >= 0 && s1c >= 0) ? s0c + p.d0.nx * s1c : -1;
   const int cboff = plane_boff<R>(coff);
   const bool out_ok = (X0 + tx < p.d0.olo + p.d0.nout) && (Y0 + ty < p.d1.olo + p.d1.nout);
-  auto col_at = [&](int z) -> R { return plane_ld<R>(plane_rsrc(xa, sxy, fsrc(p.d2, z)), cboff); };
 
   // ---- halo values of this lane: element t + j NT of every plane of a step
   // (register k = q NHL + j holds plane q: the plane is known statically)
@@ -92,28 +91,45 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
       const int u = g / TX;
       r = u < HL ? u - HL : TY + (u - HL);
     }
-    hslot[j] = valid ? (r + HL) * TW + (c + HL) : -1;
+    hslot[j] = valid ? (r + HL) * TW + (c + HL) : 0;  // else the corner (0, 0): no task reads it
     const int s0 = fsrc(p.d0, X0 + c), s1 = fsrc(p.d1, Y0 + r);
     hoff[j] = plane_boff<R>((valid && s0 >= 0 && s1 >= 0) ? s0 + p.d0.nx * s1 : -1);
   }
-  auto load_halo = [&](int z0, R* dst) {  // zero padding read as out-of-range
+  // planes of the column values and halos loaded next, one cursor each
+  // (zero padding read as out-of-range)
+  ZCursor zc_col(p.d2, zb + F::LO), zc_halo(p.d2, zb);
+  auto col_next = [&]() -> R {
+    const R v = plane_ld<R>(plane_rsrc(xa, sxy, zc_col.src()), cboff);
+    zc_col.next();
+    return v;
+  };
+  auto load_halo = [&](R* dst) {
 #pragma unroll
     for (int q = 0; q < QZ; ++q) {
-      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(xa, sxy, fsrc(p.d2, z0 + q));
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(xa, sxy, zc_halo.src());
+      zc_halo.next();
 #pragma unroll
       for (int j = 0; j < NHL; ++j) dst[q * NHL + j] = plane_ld<R>(rs, hoff[j]);
     }
   };
+  // output plane z has an input value (x~ exists) on every periodic plane,
+  // else on the stored / logical range
+  const bool zper = p.d2.bc == PER;
+  const int zoff = p.d2.bc == SHRINK ? p.d2.ox : 0, zlim = p.d2.bc == SHRINK ? p.d2.nx : p.d2.n;
+  // output column of this lane and the offset of the next output plane
+  const long ocol = (X0 + tx - p.d0.oy) + (long)p.d0.ny * (Y0 + ty - p.d1.oy);
+  const long syo = (long)p.d0.ny * p.d1.ny;
+  long zo = (long)(zb - p.d2.oy) * syo;
 
   // ---- window: steps zb + LO .. zb + LO + WS - 1 of the column
   R win[WS];
 #pragma unroll
-  for (int s = 0; s < W - 1; ++s) win[s] = col_at(zb + F::LO + s);
+  for (int s = 0; s < W - 1; ++s) win[s] = col_next();  // planes zb + LO .. zb + UP - 1
   R nxt[QZ];
 #pragma unroll
-  for (int q = 0; q < QZ; ++q) nxt[q] = col_at(zb + F::UP + q);
+  for (int q = 0; q < QZ; ++q) nxt[q] = col_next();
   R hreg[NHT];
-  load_halo(zb, hreg);
+  load_halo(hreg);
 
   const bool want_dot = (p.flags & F_DOT) && arr == p.narr - 1;
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
@@ -129,13 +145,11 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
     for (int q = 0; q < QZ; ++q)
 #pragma unroll
       for (int j = 0; j < NHL; ++j)
-        if (hslot[j] >= 0) T[q * TH * TW + hslot[j]] = hreg[q * NHL + j];
-    // prefetch the next step (planes beyond the chunk are never used)
-    if (z0 + QZ < ze) {
+        T[q * TH * TW + hslot[j]] = hreg[q * NHL + j];
+    // prefetch the next step (after the last step: planes nobody uses)
 #pragma unroll
-      for (int q = 0; q < QZ; ++q) nxt[q] = col_at(z0 + QZ + F::UP + q);
-      load_halo(z0 + QZ, hreg);
-    }
+    for (int q = 0; q < QZ; ++q) nxt[q] = col_next();
+    load_halo(hreg);
     __syncthreads();
     // ---- one task per lane: x-outputs of a row or y-outputs of a column
     if (t < G::NXT) {
@@ -185,7 +199,7 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
 #pragma unroll
         for (int k = 0; k < W; ++k) zt = fmar(win[q + k], tap<R, F>(k), zt);
         const R xc = win[HL + q];
-        const bool has_x = out_ok && coff >= 0 && fsrc(p.d2, z) >= 0;
+        const bool has_x = out_ok && coff >= 0 && (zper || (unsigned)(z - zoff) < (unsigned)zlim);
         R t0 = SX[(q * TY + ty) * SW + tx] * p.a0;
         const R t1 = SY[(q * TY + ty) * SW + tx] * p.a1;
         const R t2 = zt * p.a2;
@@ -198,11 +212,11 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
         R tot = t1 + t0;
         tot = t2 + tot;
         if (out_ok) {
-          R* yp = ya + (X0 + tx - p.d0.oy) +
-                  (long)p.d0.ny * ((Y0 + ty - p.d1.oy) + (long)p.d1.ny * (z - p.d2.oy));
+          R* yp = ya + (ocol + zo);
           if (p.flags & F_AY) tot = fmar(p.a_y, *yp, tot);
           *yp = tot;
         }
+        zo += syo;
       }
     }
     // slide the window by QZ steps

@@ -25,6 +25,19 @@ __device__ __forceinline__ int fsrc(const FDim& g, int j) {
   return (s >= 0 && s < g.nx) ? s : -1;  // beyond the stored input: never used
 }
 
+// fsrc for j, j + 1, j + 2, ... without a division: `raw` is j wrapped on a
+// periodic dim, else j less the first stored index; a z-marching kernel keeps
+// one cursor per plane stream in scalar registers
+struct ZCursor {
+  int raw, lim, n;
+  bool per;
+  __device__ __forceinline__ ZCursor(const FDim& g, int j)
+      : raw(g.bc == PER ? pmod(j, g.n) : j - (g.bc == SHRINK ? g.ox : 0)),
+        lim(g.bc == SHRINK ? g.nx : g.n), n(g.n), per(g.bc == PER) {}
+  __device__ __forceinline__ int src() const { return (unsigned)raw < (unsigned)lim ? raw : -1; }
+  __device__ __forceinline__ void next() { raw = (per && raw + 1 == n) ? 0 : raw + 1; }
+};
+
 // One input plane as a raw buffer resource (stride 0, 32-bit records).  A
 // buffer load whose byte offset is >= the record count reads zero, so a tile
 // value outside the input (zero padding of the grow / shrink conditions) is
