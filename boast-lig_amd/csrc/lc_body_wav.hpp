@@ -106,12 +106,11 @@ __device__ __forceinline__ void sepw3d_body(const FusedArgs<R>& p, int bx, int b
     const int j0 = fsrc(p.d0, X0 + LO + s0), j1 = fsrc(p.d1, Y0 + LO + s1);
     const bool ok = e < NA && j0 >= 0 && j1 >= 0;
     aoff[k] = plane_boff<R>(ok ? (int)(in_elem<K>(j0, v0, p.d0.nx) + rs_in * in_elem<K>(j1, v1, p.d1.nx)) : -1);
-    aslot[k] = e < NA ? rr * AROW + v0 * TWS + s0 : -1;
+    aslot[k] = e < NA ? rr * AROW + v0 * TWS + s0 : 2 * TWS;  // else row 0's padding: never read
   }
-  // the two z-planes of input step j2 as buffer resources: zero padding is
+  // the two z-planes of input step s2 as buffer resources: zero padding is
   // read as out-of-range (no branch, no select on the loaded values)
-  auto load_step = [&](int j2, R (*dst)[NAT]) {
-    const int s2 = fsrc(p.d2, j2);  // uniform
+  auto load_src = [&](int s2, R (*dst)[NAT]) {  // s2 uniform
     const long np2 = s2 >= 0 ? ps_in : 0;
     const __amdgpu_buffer_rsrc_t r0 = span_rsrc(xa + (s2 >= 0 ? ps_in * in_elem<K>(s2, 0, p.d2.nx) : 0), np2);
     const __amdgpu_buffer_rsrc_t r1 = span_rsrc(xa + (s2 >= 0 ? ps_in * in_elem<K>(s2, 1, p.d2.nx) : 0), np2);
@@ -127,17 +126,20 @@ __device__ __forceinline__ void sepw3d_body(const FusedArgs<R>& p, int bx, int b
   const int pfirst = zb + LO;
 #pragma unroll
   for (int h = 0; h < AD; ++h)
-    if (pfirst + h <= plast) load_step(pfirst + h, areg[h]);
+    if (pfirst + h <= plast) load_src(fsrc(p.d2, pfirst + h), areg[h]);
+  // source of the input step loaded next (every filter step loads the step AD
+  // ahead; past the last one it loads a step nobody uses)
+  ZCursor zc_ld(p.d2, pfirst + AD);
 
-  // x and y passes of input z-step j2 (tile in ar) -> pv[c2][c1']
-  auto filter_step = [&](int j2, R (*ar)[NAT], R (*pv)[2]) {
+  // x and y passes of the next input z-step (tile in ar) -> pv[c2][c1']
+  auto filter_step = [&](R (*ar)[NAT], R (*pv)[2]) {
 #pragma unroll
-    for (int k = 0; k < NAT; ++k)
-      if (aslot[k] >= 0) {
-        A[0][aslot[k]] = ar[0][k];
-        A[1][aslot[k]] = ar[1][k];
-      }
-    if (j2 + AD <= plast) load_step(j2 + AD, ar);
+    for (int k = 0; k < NAT; ++k) {
+      A[0][aslot[k]] = ar[0][k];
+      A[1][aslot[k]] = ar[1][k];
+    }
+    load_src(zc_ld.src(), ar);
+    zc_ld.next();
     __syncthreads();
     for (int task = t; task < NXT; task += NT) {
       // consecutive lanes on consecutive rows (odd stride: no bank conflicts)
@@ -167,7 +169,7 @@ __device__ __forceinline__ void sepw3d_body(const FusedArgs<R>& p, int bx, int b
       const int q = q0 + j;
       if (q < W - 1) {
         R pv[2][2];
-        filter_step(pfirst + q, areg[j], pv);
+        filter_step(areg[j], pv);  // input step pfirst + q
 #pragma unroll
         for (int c1 = 0; c1 < 2; ++c1) {
 #pragma unroll
@@ -187,7 +189,7 @@ __device__ __forceinline__ void sepw3d_body(const FusedArgs<R>& p, int bx, int b
       const int z = z0 + u;
       if (z < ze) {
         R pv[2][2];
-        filter_step(z + UP, areg[((W - 1) % AD + u) % AD], pv);
+        filter_step(areg[((W - 1) % AD + u) % AD], pv);  // input step z + UP
         const long zo0 = ps_out * out_elem<K>(z - p.d2.oy, 0, p.d2.ny);
         const long zo1 = ps_out * out_elem<K>(z - p.d2.oy, 1, p.d2.ny);
 #pragma unroll
