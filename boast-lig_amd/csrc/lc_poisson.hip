@@ -18,12 +18,17 @@
 //   div_u_i:             du = sum_k D_k u_k / h_k, optional cc
 //   nabla_u:             du_k = D_k u / h_k
 //   nabla_u_and_square:  the same and du2 = |du|^2
-// The derivative sums run in increasing tap order, as the reference loops.
+// The derivative sums run in increasing tap order, as the reference loops;
+// the 1 / h_k factors are products with the rounded reciprocals (within an
+// ulp of the quotients: a division is a dozen FP64 instructions).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
+#include <stdlib.h>
+
 #include "lc_common.hpp"
+#include "lc_fused.hpp"
 #include "lc_kernels.hpp"
 #include "lc_taps.h"
 
@@ -43,8 +48,29 @@ __constant__ float kPoiF4[] = {LC_POISSON_4(LC_PQF)};
 __constant__ float kPoiF6[] = {LC_POISSON_6(LC_PQF)};
 __constant__ float kPoiF8[] = {LC_POISSON_8(LC_PQF)};
 __constant__ float kPoiF16[] = {LC_POISSON_16(LC_PQF)};
+// the same rows as compile-time constants: the z-marching kernels' centred
+// taps then become instruction operands, not loads
+template <int NORD>
+struct PoiRows;
+#define LC_POI_ROWS(NO)                                       \
+  template <>                                                 \
+  struct PoiRows<NO> {                                        \
+    static constexpr double d[] = {LC_POISSON_##NO(LC_PQD)};  \
+    static constexpr float f[] = {LC_POISSON_##NO(LC_PQF)};   \
+  };
+LC_POI_ROWS(2)
+LC_POI_ROWS(4)
+LC_POI_ROWS(6)
+LC_POI_ROWS(8)
+LC_POI_ROWS(16)
+#undef LC_POI_ROWS
 #undef LC_PQD
 #undef LC_PQF
+
+template <typename R, int NORD>
+__device__ __forceinline__ constexpr R prow(int k) {
+  if constexpr (sizeof(R) == 8) return PoiRows<NORD>::d[k]; else return PoiRows<NORD>::f[k];
+}
 
 template <typename R, int NORD>
 __device__ __forceinline__ const R* ptab() {
@@ -94,6 +120,7 @@ struct PGeom {
   int n0, n1, n2;
   long np;
   bool per0, per1, per2;
+  int z0, nz;  // planes the one-point kernels compute: z0 .. z0 + nz - 1
 };
 
 __device__ __forceinline__ void pcoords(const PGeom& g, long p, int& i0, int& i1, int& i2) {
@@ -124,9 +151,9 @@ __device__ __forceinline__ bool pblock(const PGeom& g, int segs, long& p0, int& 
   const long l = b >> 3;
   const int lo = (int)((long)segs * x / 8), hi = (int)((long)segs * (x + 1) / 8);
   const int c = hi - lo;
-  if (c <= 0 || l >= (long)c * g.n2) return false;
+  if (c <= 0 || l >= (long)c * g.nz) return false;
   const long plane = (long)g.n0 * g.n1;
-  const int z = (int)(l / c), seg = lo + (int)(l % c);
+  const int z = g.z0 + (int)(l / c), seg = lo + (int)(l % c);
   p0 = (long)z * plane + (long)seg * kPThreads;
   count = (int)std::min<long>(kPThreads, plane - (long)seg * kPThreads);
   return true;
@@ -148,7 +175,7 @@ __global__ __launch_bounds__(kPThreads) void poisson_1d_kernel(PGeom g, int segs
 
 template <typename R, int NORD>
 __global__ __launch_bounds__(kPThreads) void div_u_i_kernel(PGeom g, int segs, const R* __restrict__ u,
-                                                            R* __restrict__ du, R* __restrict__ cc, R h0, R h1, R h2) {
+                                                            R* __restrict__ du, R* __restrict__ cc, R rh0, R rh1, R rh2) {
   long p0;
   int count;
   if (!pblock(g, segs, p0, count) || (int)threadIdx.x >= count) return;
@@ -158,41 +185,273 @@ __global__ __launch_bounds__(kPThreads) void div_u_i_kernel(PGeom g, int segs, c
   const R* u2 = u + 2 * g.np;
   int i0, i1, i2;
   pcoords(g, p, i0, i1, i2);
-  const R dx = dk<R, NORD>(g, u0, p, 0, i0, i1, i2) / h0;
-  const R dy = dk<R, NORD>(g, u1, p, 1, i0, i1, i2) / h1;
-  const R dz = dk<R, NORD>(g, u2, p, 2, i0, i1, i2) / h2;
+  const R dx = dk<R, NORD>(g, u0, p, 0, i0, i1, i2) * rh0;
+  const R dy = dk<R, NORD>(g, u1, p, 1, i0, i1, i2) * rh1;
+  const R dz = dk<R, NORD>(g, u2, p, 2, i0, i1, i2) * rh2;
   du[p] = (dx + dy) + dz;
   if (cc) {  // uniform
     const R x = u0[p], y = u1[p], z = u2[p];
-    const R xy = dk<R, NORD>(g, u1, p, 0, i0, i1, i2) / h0;
-    const R xz = dk<R, NORD>(g, u2, p, 0, i0, i1, i2) / h0;
-    const R yz = dk<R, NORD>(g, u2, p, 1, i0, i1, i2) / h1;
+    const R xy = dk<R, NORD>(g, u1, p, 0, i0, i1, i2) * rh0;
+    const R xz = dk<R, NORD>(g, u2, p, 0, i0, i1, i2) * rh0;
+    const R yz = dk<R, NORD>(g, u2, p, 1, i0, i1, i2) * rh1;
     cc[p] = (((x * x) * dx + (y * y) * dy) + (z * z) * dz) + R(2) * ((x * y * xy + x * z * xz) + y * z * yz);
   }
 }
 
 template <typename R, int NORD>
 __global__ __launch_bounds__(kPThreads) void nabla_u_kernel(PGeom g, int segs, const R* __restrict__ u,
-                                                            R* __restrict__ du, R* __restrict__ du2, R h0, R h1, R h2) {
+                                                            R* __restrict__ du, R* __restrict__ du2, R rh0, R rh1, R rh2) {
   long p0;
   int count;
   if (!pblock(g, segs, p0, count) || (int)threadIdx.x >= count) return;
   const long p = p0 + threadIdx.x;
   int i0, i1, i2;
   pcoords(g, p, i0, i1, i2);
-  const R dx = dk<R, NORD>(g, u, p, 0, i0, i1, i2) / h0;
-  const R dy = dk<R, NORD>(g, u, p, 1, i0, i1, i2) / h1;
-  const R dz = dk<R, NORD>(g, u, p, 2, i0, i1, i2) / h2;
+  const R dx = dk<R, NORD>(g, u, p, 0, i0, i1, i2) * rh0;
+  const R dy = dk<R, NORD>(g, u, p, 1, i0, i1, i2) * rh1;
+  const R dz = dk<R, NORD>(g, u, p, 2, i0, i1, i2) * rh2;
   du[p] = dx;
   du[g.np + p] = dy;
   du[2 * g.np + p] = dz;
   if (du2) du2[p] = (dx * dx + dy * dy) + dz * dz;  // uniform branch
 }
 
+// ------------------------------------------------------------ z-marching form
+// div_u_i and nabla_u as one z march per TX x TY tile of output columns.  Per
+// plane z the tiles of the inputs (the tile plus an M-point halo along x, and
+// along y where a y-derivative is taken) go to LDS from registers loaded two
+// planes ahead; every lane owns one output column, takes its x- and
+// y-stencils from LDS and the z-stencil of the last input from a register
+// window of its column along z.  Each input value is read from HBM about
+// once, and the stencil reads hit LDS, not L2.
+//
+// Non-periodic borders: an output within M points of the border uses the
+// shifted row on inputs 0 .. 2m or n - 1 - 2m .. n - 1.  A tile holding such
+// outputs along x or y shifts its LDS window to [0, T + 2m) or
+// [n - T - 2m, n), which holds every input its outputs use, and its border
+// lanes read their row from LDS too (the rows are copied to LDS once).  On a
+// non-periodic z the 2m border planes are left to the one-point kernels, and
+// the march covers planes m .. n - m - 1.  Taps, order and operations equal
+// pderiv's, so both forms give identical bits.
+constexpr int kZTX = 32, kZTY = 8, kZNT = kZTX * kZTY;
+
+// MODE 0: div_u_i, 1: div_u_i with cc, 2: nabla_u (du2 when given).
+// Derivatives taken of input k (X, Y in LDS; Z in the register window).
+template <int MODE, int K>
+struct PNeed {
+  static constexpr bool X = MODE == 0 ? K == 0 : true;
+  static constexpr bool Y = MODE == 0 ? K == 1 : (MODE == 1 ? K >= 1 : true);
+  static constexpr bool TILE = X || Y || MODE == 1;  // cc also needs the centre values
+};
+
+template <typename R, int NORD>
+struct PzGeo {
+  static constexpr int M = NORD / 2, W = NORD + 1;
+  static constexpr int TWP = (kZTX + 2 * M) | 1, TH = kZTY + 2 * M;
+  static constexpr int TSZ = TH * TWP + 1;  // one tile plus a spare slot
+  static constexpr int nload(bool x, bool y) { return (kZTY + (y ? 2 * M : 0)) * (kZTX + (x ? 2 * M : 0)); }
+  static constexpr int nlane(bool x, bool y) { return (nload(x, y) + kZNT - 1) / kZNT; }
+};
+
+// first index of a tile's LDS window along one dim: the centred halo, or on
+// a non-periodic dim the window [0, T + 2m) / [n - T - 2m, n) of a tile that
+// holds border outputs (the low rule wins when both apply)
+__device__ __forceinline__ int pz_window(int o0, int t, int m, int n, bool per) {
+  if (per) return o0 - m;
+  if (o0 < m) return 0;
+  if (o0 + t > n - m) return n - t - 2 * m;
+  return o0 - m;
+}
+
+// the lane's entries of input k's tile: LDS slot and element offset in a plane
+template <typename R, int NORD, int MODE, int K>
+struct PzTile {
+  using G = PzGeo<R, NORD>;
+  using N = PNeed<MODE, K>;
+  static constexpr int NL = N::TILE ? G::nlane(N::X, N::Y) : 0;
+  int slot[NL > 0 ? NL : 1], boff[NL > 0 ? NL : 1];
+  R val[2][NL > 0 ? NL : 1];  // two planes in flight
+  __device__ __forceinline__ void init(const PGeom& g, int X0, int Y0, int wx0, int wy0, int t) {
+    constexpr int M = G::M;
+    constexpr int NC = kZTX + (N::X ? 2 * M : 0), NLD = G::nload(N::X, N::Y);
+    const int rx0 = N::X ? wx0 : X0, ry0 = N::Y ? wy0 : Y0;  // region origin
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      const int e = t + j * kZNT;
+      const int gx = rx0 + e % NC, gy = ry0 + e / NC;
+      const int sx = g.per0 ? pmod(gx, g.n0) : gx, sy = g.per1 ? pmod(gy, g.n1) : gy;
+      // on a non-periodic dim only [0, n) is ever read
+      const bool in = e < NLD && sx >= 0 && sx < g.n0 && sy >= 0 && sy < g.n1;
+      slot[j] = in ? (gy - wy0) * G::TWP + (gx - wx0) : G::TSZ - 1;
+      boff[j] = plane_boff<R>(in ? sx + g.n0 * sy : -1);
+    }
+  }
+  __device__ __forceinline__ void load(int b, const R* u, long plane, int s2) {
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(u, plane, s2);
+#pragma unroll
+    for (int j = 0; j < NL; ++j) val[b][j] = plane_ld<R>(rs, boff[j]);
+  }
+  __device__ __forceinline__ void store(int b, R* T) const {
+#pragma unroll
+    for (int j = 0; j < NL; ++j) T[slot[j]] = val[b][j];
+  }
+};
+
+// waves per SIMD asked of the compiler: 4 (<= 128 VGPRs), 2 for the FP64 form
+// with cc, whose three tiles and six stencils would spill at 128 or 168
+template <typename R, int MODE>
+constexpr int pz_waves() { return MODE == 1 && sizeof(R) == 8 ? 2 : 4; }
+
+template <typename R, int NORD, int MODE>
+__global__ __launch_bounds__(kZNT) __attribute__((amdgpu_waves_per_eu(pz_waves<R, MODE>()))) void poisson_zm_kernel(
+    PGeom g, int tiles_x, int tiles, int zlo, int zhi, int zchunk, const R* __restrict__ u, R* __restrict__ o0,
+    R* __restrict__ o1, R rh0, R rh1, R rh2) {
+  using G = PzGeo<R, NORD>;
+  constexpr int M = G::M, W = G::W, TWP = G::TWP, TSZ = G::TSZ;
+  constexpr int NI = MODE == 2 ? 1 : 3;
+  __shared__ __align__(16) R lds[NI * TSZ];
+  __shared__ R rows[W * W];  // every coefficient row (border lanes index it)
+  for (int k = threadIdx.x; k < W * W; k += kZNT) rows[k] = prow<R, NORD>(k);
+  const int lin = xcd_linear();
+  const int tile = lin % tiles, zc = lin / tiles;
+  const int X0 = (tile % tiles_x) * kZTX, Y0 = (tile / tiles_x) * kZTY;
+  const int zb = zlo + zc * zchunk, ze = min(zb + zchunk, zhi);
+  const int t = threadIdx.x, tx = t % kZTX, ty = t / kZTX;
+  const int i0 = X0 + tx, i1 = Y0 + ty;
+  const bool out_ok = i0 < g.n0 && i1 < g.n1;
+  const long plane = (long)g.n0 * g.n1;
+  const R* uz = u + (NI - 1) * g.np;  // the input taken along z
+  const int wx0 = pz_window(X0, kZTX, M, g.n0, g.per0), wy0 = pz_window(Y0, kZTY, M, g.n1, g.per1);
+
+  // the lane's stencils in the LDS window: first tap (row / column) and
+  // coefficient row; border lanes of non-periodic dims take the shifted rows
+  const int lr = i1 - wy0, lc = i0 - wx0;
+  const bool bxl = !g.per0 && i0 < M, bxh = !g.per0 && i0 >= g.n0 - M;
+  const bool byl = !g.per1 && i1 < M, byh = !g.per1 && i1 >= g.n1 - M;
+  const int xs = (bxl ? 0 : bxh ? g.n0 - 1 - 2 * M : i0 - M) - wx0;
+  const int ys = (byl ? 0 : byh ? g.n1 - 1 - 2 * M : i1 - M) - wy0;
+  const int xrow = (bxl ? i0 : bxh ? i0 - g.n0 + 2 * M + 1 : M) * W;  // first tap of the row
+  const int yrow = (byl ? i1 : byh ? i1 - g.n1 + 2 * M + 1 : M) * W;
+  const bool bx = bxl || bxh, by = byl || byh;
+
+  PzTile<R, NORD, MODE, 0> t0;
+  PzTile<R, NORD, MODE, 1> t1;
+  PzTile<R, NORD, MODE, 2> t2;
+  t0.init(g, X0, Y0, wx0, wy0, t);
+  if constexpr (NI == 3) {
+    t1.init(g, X0, Y0, wx0, wy0, t);
+    t2.init(g, X0, Y0, wx0, wy0, t);
+  }
+  FDim dz{};
+  dz.n = g.n2; dz.nx = g.n2; dz.bc = g.per2 ? PER : FREE;
+  ZCursor zc_tile(dz, zb), zc_col(dz, zb - M);
+  auto load_tiles = [&](int b) {  // b static after unrolling
+    const int s2 = zc_tile.src();
+    t0.load(b, u, plane, s2);
+    if constexpr (NI == 3) {
+      t1.load(b, u + g.np, plane, s2);
+      t2.load(b, u + 2 * g.np, plane, s2);
+    }
+    zc_tile.next();
+  };
+  const int cboff = plane_boff<R>(out_ok ? i0 + g.n0 * i1 : -1);
+  auto col_next = [&]() -> R {
+    const R v = plane_ld<R>(plane_rsrc(uz, plane, zc_col.src()), cboff);
+    zc_col.next();
+    return v;
+  };
+  // planes z + M (column) and z (tiles) are loaded two steps ahead of use
+  R win[W];
+#pragma unroll
+  for (int k = 0; k < W - 1; ++k) win[k] = col_next();  // planes zb - M .. zb + M - 1
+  R nxt[2];
+  nxt[0] = col_next();
+  nxt[1] = col_next();
+  load_tiles(0);
+  load_tiles(1);
+
+  // x- / y-stencil of a tile at this lane (centred taps from scalar
+  // registers; the shifted border rows per lane)
+  auto sx = [&](const R* T) {
+    const R* v = T + lr * TWP + xs;
+    R s = R(0);
+    if (!bx) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) s = fmar(prow<R, NORD>(M * W + j), v[j], s);
+    } else {
+#pragma unroll
+      for (int j = 0; j < W; ++j) s = fmar(rows[xrow + j], v[j], s);
+    }
+    return s;
+  };
+  auto sy = [&](const R* T) {
+    const R* v = T + ys * TWP + lc;
+    R s = R(0);
+    if (!by) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) s = fmar(prow<R, NORD>(M * W + j), v[j * TWP], s);
+    } else {
+#pragma unroll
+      for (int j = 0; j < W; ++j) s = fmar(rows[yrow + j], v[j * TWP], s);
+    }
+    return s;
+  };
+  auto ctr = [&](const R* T) { return T[lr * TWP + lc]; };
+
+  auto step = [&](int z, int b) {  // b = (z - zb) & 1, static after unrolling
+    win[W - 1] = nxt[b];
+    t0.store(b, lds);
+    if constexpr (NI == 3) {
+      t1.store(b, lds + TSZ);
+      t2.store(b, lds + 2 * TSZ);
+    }
+    nxt[b] = col_next();
+    load_tiles(b);
+    __syncthreads();
+    if (out_ok) {
+      const long p = (long)z * plane + i0 + (long)g.n0 * i1;
+      R dzv = R(0);
+#pragma unroll
+      for (int j = 0; j < W; ++j) dzv = fmar(prow<R, NORD>(M * W + j), win[j], dzv);
+      if constexpr (MODE == 2) {
+        const R dx = sx(lds) * rh0, dy = sy(lds) * rh1, dz2 = dzv * rh2;
+        o0[p] = dx;
+        o0[g.np + p] = dy;
+        o0[2 * g.np + p] = dz2;
+        if (o1) o1[p] = (dx * dx + dy * dy) + dz2 * dz2;
+      } else {
+        const R dx = sx(lds) * rh0;
+        const R dy = sy(lds + TSZ) * rh1;
+        const R dz2 = dzv * rh2;
+        o0[p] = (dx + dy) + dz2;
+        if constexpr (MODE == 1) {
+          const R x = ctr(lds), y = ctr(lds + TSZ), zv = win[M];
+          const R xy = sx(lds + TSZ) * rh0;
+          const R xz = sx(lds + 2 * TSZ) * rh0;
+          const R yz = sy(lds + 2 * TSZ) * rh1;
+          o1[p] = (((x * x) * dx + (y * y) * dy) + (zv * zv) * dz2) +
+                  R(2) * ((x * y * xy + x * zv * xz) + y * zv * yz);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < W - 1; ++k) win[k] = win[k + 1];
+  };
+  for (int z = zb; z < ze; z += 4) {  // four steps per trip: static buffers
+    step(z, 0);
+    if (z + 1 < ze) step(z + 1, 1);
+    if (z + 2 < ze) step(z + 2, 0);
+    if (z + 3 < ze) step(z + 3, 1);
+  }
+}
+
 PGeom pgeom(const int* n, int geocode, int bc_idim, int idim) {
   PGeom g;
   g.n0 = n[0]; g.n1 = n[1]; g.n2 = n[2];
   g.np = (long)n[0] * n[1] * n[2];
+  g.z0 = 0;
+  g.nz = n[2];
   if (idim >= 0) {  // one dim, its own bc
     g.per0 = g.per1 = g.per2 = bc_idim == 0;
   } else {          // div_u_i_ref's perx / pery / perz
@@ -205,8 +464,8 @@ PGeom pgeom(const int* n, int geocode, int bc_idim, int idim) {
 
 int psegs(const PGeom& g) { return (int)(((long)g.n0 * g.n1 + kPThreads - 1) / kPThreads); }
 
-// 8 XCD shares of ceil(S / 8) segments times n2 planes
-unsigned pgrid(const PGeom& g) { return (unsigned)(8L * ((psegs(g) + 7) / 8) * g.n2); }
+// 8 XCD shares of ceil(S / 8) segments times nz planes
+unsigned pgrid(const PGeom& g) { return (unsigned)(8L * ((psegs(g) + 7) / 8) * g.nz); }
 
 }  // namespace
 
@@ -230,22 +489,85 @@ hipError_t poisson_1d(int nord, int idim, const int* n, int bc, const R* u, R* d
   return hipGetLastError();
 }
 
+// z-march launch shape: tiles x z chunks of the planes zlo .. zhi - 1 (all,
+// or on a non-periodic z the planes off its border), ~8 blocks per CU, chunks
+// no shorter than 2 stencil lengths.  False when the one-point kernels must
+// do all (LIBCONV_POISSON_POINT set, planes beyond the buffer-record limit).
+struct PzLaunch {
+  int tiles_x, tiles, zlo, zhi, zchunk;
+  unsigned blocks;
+};
+static bool pz_launch(const PGeom& g, int nord, PzLaunch& L) {
+  if (getenv("LIBCONV_POISSON_POINT")) return false;
+  if ((long)g.n0 * g.n1 * 8 > kPlaneMax) return false;
+  const int m = nord / 2;
+  L.zlo = g.per2 ? 0 : m;
+  L.zhi = g.per2 ? g.n2 : g.n2 - m;
+  const int nz = L.zhi - L.zlo;  // >= 1: n >= nord + 1
+  L.tiles_x = (g.n0 + kZTX - 1) / kZTX;
+  const long tiles = (long)L.tiles_x * ((g.n1 + kZTY - 1) / kZTY);
+  long chunks = (8L * num_cu() + tiles - 1) / tiles;
+  chunks = std::max(1L, std::min<long>(chunks, nz / (2 * (nord + 1))));
+  L.zchunk = (int)((nz + chunks - 1) / chunks);
+  const long nb = tiles * ((nz + L.zchunk - 1) / L.zchunk);
+  if (nb > 0x7fffffffL) return false;
+  L.tiles = (int)tiles;
+  L.blocks = (unsigned)nb;
+  return true;
+}
+
+// the planes of a non-periodic z the z march leaves out: 0 .. m - 1 and
+// n - m .. n - 1, as two one-point launches (g.z0 / g.nz select the planes)
+template <typename F>
+static void pz_border_planes(const PGeom& g, int nord, F&& launch) {
+  if (g.per2) return;
+  PGeom b = g;
+  b.nz = nord / 2;
+  b.z0 = 0;
+  launch(b);
+  b.z0 = g.n2 - nord / 2;
+  launch(b);
+}
+
 template <typename R>
 hipError_t div_u_i(int geocode, const int* n, const R* u, R* du, R* cc, int nord, const double* h, hipStream_t st) {
   const PGeom g = pgeom(n, geocode, 0, -1);
   if (g.np == 0) return hipSuccess;
-  LC_POISSON_DISPATCH(nord, (div_u_i_kernel<R, NO><<<pgrid(g), kPThreads, 0, st>>>(g, psegs(g), u, du, cc, (R)h[0],
-                                                                                       (R)h[1], (R)h[2])))
-  return hipGetLastError();
+  const R r0 = (R)(1.0 / h[0]), r1 = (R)(1.0 / h[1]), r2 = (R)(1.0 / h[2]);
+  auto point = [&](const PGeom& q) -> hipError_t {
+    LC_POISSON_DISPATCH(nord, (div_u_i_kernel<R, NO><<<pgrid(q), kPThreads, 0, st>>>(q, psegs(q), u, du, cc, r0, r1, r2)))
+    return hipGetLastError();
+  };
+  PzLaunch L;
+  if (!pz_launch(g, nord, L)) return point(g);
+  if (cc) {
+    LC_POISSON_DISPATCH(nord, (poisson_zm_kernel<R, NO, 1><<<L.blocks, kZNT, 0, st>>>(
+                                  g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, du, cc, r0, r1, r2)))
+  } else {
+    LC_POISSON_DISPATCH(nord, (poisson_zm_kernel<R, NO, 0><<<L.blocks, kZNT, 0, st>>>(
+                                  g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, du, cc, r0, r1, r2)))
+  }
+  hipError_t e = hipGetLastError();
+  pz_border_planes(g, nord, [&](const PGeom& q) { if (e == hipSuccess) e = point(q); });
+  return e;
 }
 
 template <typename R>
 hipError_t nabla_u(int geocode, const int* n, const R* u, R* du, R* du2, int nord, const double* h, hipStream_t st) {
   const PGeom g = pgeom(n, geocode, 0, -1);
   if (g.np == 0) return hipSuccess;
-  LC_POISSON_DISPATCH(nord, (nabla_u_kernel<R, NO><<<pgrid(g), kPThreads, 0, st>>>(g, psegs(g), u, du, du2, (R)h[0],
-                                                                                       (R)h[1], (R)h[2])))
-  return hipGetLastError();
+  const R r0 = (R)(1.0 / h[0]), r1 = (R)(1.0 / h[1]), r2 = (R)(1.0 / h[2]);
+  auto point = [&](const PGeom& q) -> hipError_t {
+    LC_POISSON_DISPATCH(nord, (nabla_u_kernel<R, NO><<<pgrid(q), kPThreads, 0, st>>>(q, psegs(q), u, du, du2, r0, r1, r2)))
+    return hipGetLastError();
+  };
+  PzLaunch L;
+  if (!pz_launch(g, nord, L)) return point(g);
+  LC_POISSON_DISPATCH(nord, (poisson_zm_kernel<R, NO, 2><<<L.blocks, kZNT, 0, st>>>(
+                                g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, du, du2, r0, r1, r2)))
+  hipError_t e = hipGetLastError();
+  pz_border_planes(g, nord, [&](const PGeom& q) { if (e == hipSuccess) e = point(q); });
+  return e;
 }
 
 #undef LC_POISSON_DISPATCH

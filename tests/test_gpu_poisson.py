@@ -126,3 +126,34 @@ def test_rejects_bad_orders_and_small_grids(lc, gpu):
         lc.div_u_i(1, [10, 10, 10], u, du, 16, [1.0] * 3)      # grid 10 < 17
     with pytest.raises(lc.LibconvError):
         lc.poisson_1d(4, 0, [10, 10, 10], 1, u, du)            # bc 1
+
+
+@pytest.mark.parametrize('nord', [4, 16])
+@pytest.mark.parametrize('geocode', [1, 2, 3])
+@pytest.mark.parametrize('dtype_name', ['float64', 'float32'])
+def test_z_march_equals_point_kernels(lc, gpu, monkeypatch, nord, geocode, dtype_name):
+    """The z-marching kernels (tiles over several blocks, partial edge tiles,
+    several z chunks, border lanes and planes) give the bits of the one-point
+    kernels (LIBCONV_POISSON_POINT) for div_u_i with and without cc and for
+    nabla_u with du2."""
+    torch = gpu
+    dtype = getattr(torch, dtype_name)
+    rng = np.random.default_rng(3000 + nord + geocode)
+    n = [70, 37, 2 * nord + 45]
+    np_ = int(np.prod(n))
+    h = [0.15, 0.2, 0.25]
+    tu = torch.from_numpy(rng.uniform(-1, 1, 3 * np_)).to('cuda', dtype)
+
+    def run():
+        out = [torch.empty(k * np_, dtype=dtype, device='cuda') for k in (1, 1, 1, 3, 1)]
+        lc.div_u_i(geocode, n, tu, out[0], nord, h, out[1])
+        lc.div_u_i(geocode, n, tu, out[2], nord, h)
+        lc.nabla_u_and_square(geocode, n, tu[:np_], out[3], out[4], nord, h)
+        torch.cuda.synchronize()
+        return out
+
+    zm = run()
+    monkeypatch.setenv('LIBCONV_POISSON_POINT', '1')
+    pt = run()
+    for a, b in zip(zm, pt):
+        assert torch.equal(a, b)
