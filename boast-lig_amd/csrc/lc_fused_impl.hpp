@@ -345,7 +345,8 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(16, 32, 16, 4, 4, 2, 4, 0, 0) X(17, 32, 16, 4, 4, 2, 2, 0, 0) X(18, 64, 8, 4, 4, 2, 4, 0, 0)       \
   X(19, 32, 16, 8, 8, 4, 2, 0, 0) X(20, 32, 16, 4, 4, 2, 4, 8, 16) X(21, 32, 16, 8, 8, 4, 2, 8, 32)    \
   X(22, 16, 16, 4, 4, 2, 4, 8, 16) X(23, 16, 16, 4, 4, 2, 2, 0, 16) X(24, 32, 8, 4, 4, 2, 3, 0, 16)    \
-  X(25, 32, 8, 4, 4, 2, 2, 0, 16) X(26, 16, 16, 4, 4, 2, 2, 0, 64) X(27, 16, 16, 4, 4, 2, 4, 8, 64)
+  X(25, 32, 8, 4, 4, 2, 2, 0, 16) X(26, 16, 16, 4, 4, 2, 2, 0, 64) X(27, 16, 16, 4, 4, 2, 4, 8, 64)    \
+  X(28, 32, 16, 8, 8, 2, 2, 0, 0) X(29, 32, 16, 8, 8, 2, 2, 0, 32) X(30, 32, 16, 8, 8, 2, 2, 8, 32)
 // two-plane separable form (sep only): tile, QX / QY outputs per task, QZ
 // planes per step, waves per EU, shortest z chunk (0: 4 W, ~2 blocks per CU;
 // else chunks down to ZC planes for ~4 blocks per CU), prefetch depth in steps
@@ -364,7 +365,7 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(37, 32, 16, 4, 2, 2, 4, 0) X(38, 32, 16, 4, 2, 2, 6, 0) X(39, 32, 16, 4, 4, 2, 4, 0)            \
   X(40, 32, 16, 4, 2, 1, 6, 0) X(41, 32, 16, 4, 2, 4, 4, 0) X(42, 32, 16, 8, 2, 2, 4, 0)            \
   X(43, 32, 32, 4, 2, 2, 4, 0) X(44, 32, 16, 4, 2, 2, 4, 32)
-constexpr int kFusedVariantCount = 28;
+constexpr int kFusedVariantCount = 31;
 constexpr int kSepVariantCount = 45;
 
 template <typename R, class F>
