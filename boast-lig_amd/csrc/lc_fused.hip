@@ -164,7 +164,7 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     // defaults: the fastest variants at 256^3 on MI355X (tuner sweep); the
     // two-plane forms (16 and up) take planes of at most kPlaneMax bytes
     const bool fit = plane_fits(b, prec);
-    if (form == FORM_SEP) return !fit ? 12 : 37;
+    if (form == FORM_SEP) return !fit ? 12 : 45;
     if (!fit) return prec == 4 ? 7 : 8;
     return prec == 4 ? 21 : 19;
   }
