@@ -24,7 +24,7 @@ def test_variant_counts(lc):
 
 
 def test_default_kernel_names(lc):
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<double, lc::FD2, 32, 16, 8, 8, 4, 2, 0>'
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<double, lc::FD2, 32, 16, 8, 8, 2, 2, 0>'
     assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<float, lc::FD2, 32, 16, 8, 8, 4, 2, 8>'
     assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
         'sepp_fused<float, lc::FMD, 32, 16, 4, 2, 2, 4>'
@@ -48,7 +48,7 @@ def test_environment_selects_variant(lc, monkeypatch):
     monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', '6')
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_ws<double, lc::FD2, 64, 8, 4, 4, 2>'
     monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', '99')   # out of range: default
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('32, 16, 8, 8, 4, 2, 0>')
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('32, 16, 8, 8, 2, 2, 0>')
 
 
 def test_json_cache_round_trip(lc, tmp_path, monkeypatch):
