@@ -376,9 +376,9 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(37, 32, 16, 4, 2, 2, 4, 0) X(38, 32, 16, 4, 2, 2, 6, 0) X(39, 32, 16, 4, 4, 2, 4, 0)            \
   X(40, 32, 16, 4, 2, 1, 6, 0) X(41, 32, 16, 4, 2, 4, 4, 0) X(42, 32, 16, 8, 2, 2, 4, 0)            \
   X(43, 32, 32, 4, 2, 2, 4, 0) X(44, 32, 16, 4, 2, 2, 4, 32) X(45, 32, 16, 4, 2, 2, 4, -2)          \
-  X(46, 32, 16, 4, 2, 2, 6, -2) X(47, 32, 16, 4, 2, 2, 4, -3)
+  X(46, 32, 16, 4, 2, 2, 6, -2) X(47, 32, 16, 4, 2, 2, 4, -3) X(48, 32, 16, 4, 2, 2, 4, -4)
 constexpr int kFusedVariantCount = 31;
-constexpr int kSepVariantCount = 48;
+constexpr int kSepVariantCount = 49;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,

@@ -25,7 +25,7 @@ def lc(gpu):
     libconv.tune_clear()
 
 
-@pytest.mark.parametrize('v', range(48))
+@pytest.mark.parametrize('v', range(49))
 @pytest.mark.parametrize('bc', [(1, -1, 0), (2, 0, 2)])
 def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v, bc):
     torch = gpu
