@@ -415,8 +415,8 @@ def main():
         print(json.dumps({'copy_calib_bytes_read': nb, 'copy_calib_bytes_written': nb, 'launches': 3}))
         return 0
     if args.gpus != 1:
-        print('bench.py: this build measures one GPU; --gpus %d is not supported '
-              '(orbitals shard across GPUs without communication, see DESIGN.md)' % args.gpus)
+        print('bench.py: measures one GPU; --gpus %d is not supported (orbitals would shard '
+              'across GPUs without communication, a single grid by z slabs with halo planes)' % args.gpus)
         return 2
     import numpy as np
     import torch

@@ -1,6 +1,6 @@
 """GPU parity: the HIP library through its C ABI against the FP64 oracle.
 
-Bar: FP64 within 1e-13 relative max-norm of the oracle (the north-star
+Bar: FP64 within 1e-13 relative max-norm of the oracle (the path's stated
 tolerance; the reference's own tests use 1e-14 for MagicFilter, 1e-12 for
 Kinetic, 1e-11 for wavelets); FP32 storage and arithmetic within 2e-6
 (MagicFilter / wavelets) and 1e-5 (Kinetic, whose sum of |taps| is 9.9) of
