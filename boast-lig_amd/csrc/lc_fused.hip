@@ -169,7 +169,7 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     return prec == 4 ? 21 : 29;
   }
   v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
-  return v >= 0 ? v : 3;  // one z-step per window slide: as fast, fewer registers
+  return v >= 0 ? v : 5;  // one z-step per slide, at most two blocks per CU
 }
 
 template <typename R>

@@ -52,12 +52,14 @@ __global__ __launch_bounds__(2 * TX * TY) __attribute__((amdgpu_waves_per_eu(WPE
 
 // ====================================================================== host
 // Compiled variants: tile TX x TY steps, QX outputs per x-task, BZ steps per
-// window slide, AD prefetch depth (2 TX TY lanes per block), waves per EU.
-#define LC_WAV_VARIANTS(X)                                                                  \
-  X(0, 16, 8, 4, 2, 2, 1) X(1, 8, 8, 4, 2, 2, 1) X(2, 32, 4, 4, 2, 2, 1) X(3, 16, 8, 4, 1, 1, 1) \
-  X(4, 16, 8, 4, 1, 1, 2)
+// window slide, AD prefetch depth (2 TX TY lanes per block), waves per EU,
+// z chunks of a single call (0: ~2 blocks per CU, chunks >= 2 W steps;
+// ZC < 0: at most -ZC blocks on any CU, chunks >= W steps)
+#define LC_WAV_VARIANTS(X)                                                                            \
+  X(0, 16, 8, 4, 2, 2, 1, 0) X(1, 8, 8, 4, 2, 2, 1, 0) X(2, 32, 4, 4, 2, 2, 1, 0) X(3, 16, 8, 4, 1, 1, 1, 0) \
+  X(4, 16, 8, 4, 1, 1, 2, 0) X(5, 16, 8, 4, 1, 1, 1, -2) X(6, 16, 8, 4, 2, 2, 1, -2)
 
-int wav_variant_count() { return 5; }
+int wav_variant_count() { return 7; }
 
 std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
   (void)id;
@@ -67,7 +69,7 @@ std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
   char buf[200];
   buf[0] = 0;
   switch (v) {
-#define X(I, TX, TY, QX, BZ, AD, WPE)                                                                   \
+#define X(I, TX, TY, QX, BZ, AD, WPE, ZC)                                                                   \
   case I:                                                                                         \
     snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>", batch ? "sepw3d_batch" : "sepw3d_fused", \
              r, op, TX, TY, QX, BZ, AD, WPE);                                                        \
@@ -95,14 +97,23 @@ static FusedArgs<R> wav_args(const Box3& b, int narr, const R* x, R* y, R a, R a
   return p;
 }
 
-template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD, int WPE>
+template <typename R, class Op, int TX, int TY, int QX, int BZ, int AD, int WPE, int ZC>
 static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
   if (!wav_plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, BZ, AD>(b, narr, x, y, a, a_y);
   const long tiles = (long)p.tiles_x * p.tiles_y;
-  // z chunks no shorter than 2 W steps (W steps when the grid is too small
-  // to give every CU a block)
-  p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), std::max(1, Op::W / 2), num_cu());
+  if (ZC < 0) {
+    // at most -ZC blocks on any CU, at least one where the grid allows
+    const long per = tiles * std::max(narr, 1);
+    long chunks = (long)(-ZC) * num_cu() / per;
+    if (chunks * per < num_cu()) chunks = (num_cu() + per - 1) / per;
+    chunks = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / Op::W)));
+    p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
+  } else {
+    // z chunks no shorter than 2 W steps (W steps when the grid is too
+    // small to give every CU a block)
+    p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), std::max(1, Op::W / 2), num_cu());
+  }
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
@@ -114,8 +125,8 @@ template <typename R, class Op>
 static hipError_t sepw3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
                                  hipStream_t st) {
   switch (v) {
-#define X(I, TX, TY, QX, BZ, AD, WPE) \
-  case I: return launch_sepw3d<R, Op, TX, TY, QX, BZ, AD, WPE>(b, narr, x, y, a, a_y, st);
+#define X(I, TX, TY, QX, BZ, AD, WPE, ZC) \
+  case I: return launch_sepw3d<R, Op, TX, TY, QX, BZ, AD, WPE, ZC>(b, narr, x, y, a, a_y, st);
     LC_WAV_VARIANTS(X)
 #undef X
   }
@@ -175,7 +186,7 @@ template <typename R, class Op>
 static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, BatchStage* s, hipStream_t st) {
   if (build) {
     switch (s->variant) {
-#define X(I, TX, TY, QX, BZ, AD, WPE) \
+#define X(I, TX, TY, QX, BZ, AD, WPE, ZC) \
   case I: return wav_stage_build<R, Op, TX, TY, QX, BZ, AD, WPE>(it, count, s);
       LC_WAV_VARIANTS(X)
 #undef X
@@ -185,7 +196,7 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
   if (s->blocks == 0) return hipSuccess;
   const FusedArgs<R>* tab = static_cast<const FusedArgs<R>*>(s->tab);
   switch (s->variant) {
-#define X(I, TX, TY, QX, BZ, AD, WPE)                                                                     \
+#define X(I, TX, TY, QX, BZ, AD, WPE, ZC)                                                                     \
   case I:                                                                                           \
     sepw3d_batch<R, Op, TX, TY, QX, BZ, AD, WPE><<<s->blocks, 2 * TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;

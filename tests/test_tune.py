@@ -19,8 +19,8 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 def test_variant_counts(lc):
     assert lc.variant_count(lc.FORM_SEP) == 49
     assert lc.variant_count(lc.FORM_SUM) == 31
-    assert lc.variant_count(lc.FORM_DWT) == 5
-    assert lc.variant_count(lc.FORM_IDWT) == 5
+    assert lc.variant_count(lc.FORM_DWT) == 7
+    assert lc.variant_count(lc.FORM_IDWT) == 7
 
 
 def test_default_kernel_names(lc):
