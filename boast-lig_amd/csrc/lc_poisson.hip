@@ -490,14 +490,17 @@ hipError_t poisson_1d(int nord, int idim, const int* n, int bc, const R* u, R* d
 }
 
 // z-march launch shape: tiles x z chunks of the planes zlo .. zhi - 1 (all,
-// or on a non-periodic z the planes off its border), ~8 blocks per CU, chunks
-// no shorter than 2 stencil lengths.  False when the one-point kernels must
-// do all (LIBCONV_POISSON_POINT set, planes beyond the buffer-record limit).
+// or on a non-periodic z the planes off its border).  The chunks put at most
+// `resident` blocks (what one CU holds of the kernel) on any CU, so the grid
+// runs in one even round, yet at least one block per CU where the grid
+// allows; chunks no shorter than 2 stencil lengths.  False when the
+// one-point kernels must do all (LIBCONV_POISSON_POINT set, planes beyond
+// the buffer-record limit).
 struct PzLaunch {
   int tiles_x, tiles, zlo, zhi, zchunk;
   unsigned blocks;
 };
-static bool pz_launch(const PGeom& g, int nord, PzLaunch& L) {
+static bool pz_launch(const PGeom& g, int nord, int resident, PzLaunch& L) {
   if (getenv("LIBCONV_POISSON_POINT")) return false;
   if ((long)g.n0 * g.n1 * 8 > kPlaneMax) return false;
   const int m = nord / 2;
@@ -506,7 +509,8 @@ static bool pz_launch(const PGeom& g, int nord, PzLaunch& L) {
   const int nz = L.zhi - L.zlo;  // >= 1: n >= nord + 1
   L.tiles_x = (g.n0 + kZTX - 1) / kZTX;
   const long tiles = (long)L.tiles_x * ((g.n1 + kZTY - 1) / kZTY);
-  long chunks = (8L * num_cu() + tiles - 1) / tiles;
+  long chunks = (long)std::max(1, resident) * num_cu() / tiles;
+  if (chunks * tiles < num_cu()) chunks = (num_cu() + tiles - 1) / tiles;
   chunks = std::max(1L, std::min<long>(chunks, nz / (2 * (nord + 1))));
   L.zchunk = (int)((nz + chunks - 1) / chunks);
   const long nb = tiles * ((nz + L.zchunk - 1) / L.zchunk);
@@ -514,6 +518,30 @@ static bool pz_launch(const PGeom& g, int nord, PzLaunch& L) {
   L.tiles = (int)tiles;
   L.blocks = (unsigned)nb;
   return true;
+}
+
+// blocks of one z-march kernel that a CU holds (registers, LDS), cached
+template <auto KER>
+static int pz_resident() {
+  static int nb = 0;
+  if (!nb) {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, KER, kZNT, 0) != hipSuccess || v < 1) v = 1;
+    nb = v;
+  }
+  return nb;
+}
+
+// one z-march launch; *done false when the one-point kernels must run
+template <typename R, int NO, int MODE>
+static hipError_t pz_run(const PGeom& g, const R* u, R* o0, R* o1, R r0, R r1, R r2, hipStream_t st,
+                         bool* done) {
+  PzLaunch L;
+  *done = pz_launch(g, NO, pz_resident<poisson_zm_kernel<R, NO, MODE>>(), L);
+  if (!*done) return hipSuccess;
+  poisson_zm_kernel<R, NO, MODE><<<L.blocks, kZNT, 0, st>>>(g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, o0,
+                                                              o1, r0, r1, r2);
+  return hipGetLastError();
 }
 
 // the planes of a non-periodic z the z march leaves out: 0 .. m - 1 and
@@ -538,16 +566,15 @@ hipError_t div_u_i(int geocode, const int* n, const R* u, R* du, R* cc, int nord
     LC_POISSON_DISPATCH(nord, (div_u_i_kernel<R, NO><<<pgrid(q), kPThreads, 0, st>>>(q, psegs(q), u, du, cc, r0, r1, r2)))
     return hipGetLastError();
   };
-  PzLaunch L;
-  if (!pz_launch(g, nord, L)) return point(g);
+  bool done = false;
+  hipError_t e = hipSuccess;
   if (cc) {
-    LC_POISSON_DISPATCH(nord, (poisson_zm_kernel<R, NO, 1><<<L.blocks, kZNT, 0, st>>>(
-                                  g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, du, cc, r0, r1, r2)))
+    LC_POISSON_DISPATCH(nord, (e = pz_run<R, NO, 1>(g, u, du, cc, r0, r1, r2, st, &done)))
   } else {
-    LC_POISSON_DISPATCH(nord, (poisson_zm_kernel<R, NO, 0><<<L.blocks, kZNT, 0, st>>>(
-                                  g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, du, cc, r0, r1, r2)))
+    LC_POISSON_DISPATCH(nord, (e = pz_run<R, NO, 0>(g, u, du, cc, r0, r1, r2, st, &done)))
   }
-  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (!done) return point(g);
   pz_border_planes(g, nord, [&](const PGeom& q) { if (e == hipSuccess) e = point(q); });
   return e;
 }
@@ -561,11 +588,11 @@ hipError_t nabla_u(int geocode, const int* n, const R* u, R* du, R* du2, int nor
     LC_POISSON_DISPATCH(nord, (nabla_u_kernel<R, NO><<<pgrid(q), kPThreads, 0, st>>>(q, psegs(q), u, du, du2, r0, r1, r2)))
     return hipGetLastError();
   };
-  PzLaunch L;
-  if (!pz_launch(g, nord, L)) return point(g);
-  LC_POISSON_DISPATCH(nord, (poisson_zm_kernel<R, NO, 2><<<L.blocks, kZNT, 0, st>>>(
-                                g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, du, du2, r0, r1, r2)))
-  hipError_t e = hipGetLastError();
+  bool done = false;
+  hipError_t e = hipSuccess;
+  LC_POISSON_DISPATCH(nord, (e = pz_run<R, NO, 2>(g, u, du, du2, r0, r1, r2, st, &done)))
+  if (e != hipSuccess) return e;
+  if (!done) return point(g);
   pz_border_planes(g, nord, [&](const PGeom& q) { if (e == hipSuccess) e = point(q); });
   return e;
 }
