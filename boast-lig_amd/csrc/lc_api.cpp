@@ -1050,7 +1050,7 @@ int stage_default_variant(const lc_batch_item* it, int count, int prec) {
   if (has_batch_form(form, v)) return v;
   bool fit = true;
   for (int i = 0; i < count; ++i) fit = fit && plane_fits(item_box(it[i]), prec);
-  if (form == FORM_SUM) return !fit ? 8 : prec == 4 ? 21 : 29;  // fastest batched kinetic forms
+  if (form == FORM_SUM) return !fit ? 8 : prec == 4 ? 32 : 29;  // fastest batched kinetic forms
   return form == FORM_SEP ? 12 : 0;
 }
 

@@ -166,7 +166,7 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     const bool fit = plane_fits(b, prec);
     if (form == FORM_SEP) return !fit ? 12 : 45;
     if (!fit) return prec == 4 ? 7 : 8;
-    return prec == 4 ? 21 : 29;
+    return prec == 4 ? 32 : 29;
   }
   v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
   return v >= 0 ? v : 5;  // one z-step per slide, at most two blocks per CU

@@ -168,6 +168,14 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
   const long tiles = (long)p.tiles_x * p.tiles_y;
   if (ZC == 0) {
     p.zchunk = pick_zchunk(tiles, p.d2.nout, std::max(narr, 1), F::N, num_cu());
+  } else if (ZC < 0) {
+    // at most -ZC blocks on any CU, at least one where the grid allows
+    const long per = tiles * std::max(narr, 1);
+    long chunks = (long)(-ZC) * num_cu() / per;
+    if (chunks * per < num_cu()) chunks = (num_cu() + per - 1) / per;
+    chunks = std::max(1L, std::min<long>(chunks, std::max(1, p.d2.nout / 16)));
+    p.zchunk = (int)((p.d2.nout + chunks - 1) / chunks);
+    p.zchunk = (p.zchunk + QZ - 1) / QZ * QZ;
   } else {
     // a z chunk only costs the window prefill (W - 1 column loads per lane):
     // chunks down to ZC planes put ~8 blocks on every CU, and down to 16
@@ -356,7 +364,8 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(19, 32, 16, 8, 8, 4, 2, 0, 0) X(20, 32, 16, 4, 4, 2, 4, 8, 16) X(21, 32, 16, 8, 8, 4, 2, 8, 32)    \
   X(22, 16, 16, 4, 4, 2, 4, 8, 16) X(23, 16, 16, 4, 4, 2, 2, 0, 16) X(24, 32, 8, 4, 4, 2, 3, 0, 16)    \
   X(25, 32, 8, 4, 4, 2, 2, 0, 16) X(26, 16, 16, 4, 4, 2, 2, 0, 64) X(27, 16, 16, 4, 4, 2, 4, 8, 64)    \
-  X(28, 32, 16, 8, 8, 2, 2, 0, 0) X(29, 32, 16, 8, 8, 2, 2, 0, 32) X(30, 32, 16, 8, 8, 2, 2, 8, 32)
+  X(28, 32, 16, 8, 8, 2, 2, 0, 0) X(29, 32, 16, 8, 8, 2, 2, 0, 32) X(30, 32, 16, 8, 8, 2, 2, 8, 32)   \
+  X(31, 32, 16, 8, 8, 2, 2, 0, -2) X(32, 32, 16, 8, 8, 4, 2, 8, -2)
 // two-plane separable form (sep only): tile, QX / QY outputs per task, QZ
 // planes per step, waves per EU, shortest z chunk (0: 4 W, ~2 blocks per CU;
 // else chunks down to ZC planes for ~4 blocks per CU), prefetch depth in steps
@@ -377,7 +386,7 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(40, 32, 16, 4, 2, 1, 6, 0) X(41, 32, 16, 4, 2, 4, 4, 0) X(42, 32, 16, 8, 2, 2, 4, 0)            \
   X(43, 32, 32, 4, 2, 2, 4, 0) X(44, 32, 16, 4, 2, 2, 4, 32) X(45, 32, 16, 4, 2, 2, 4, -2)          \
   X(46, 32, 16, 4, 2, 2, 6, -2) X(47, 32, 16, 4, 2, 2, 4, -3) X(48, 32, 16, 4, 2, 2, 4, -4)
-constexpr int kFusedVariantCount = 31;
+constexpr int kFusedVariantCount = 33;
 constexpr int kSepVariantCount = 49;
 
 template <typename R, class F>
