@@ -161,8 +161,10 @@ int select_variant(int prec, int form, int id, const Box3& b) {
   if (form == FORM_SUM || form == FORM_SEP) {
     v = env_variant(form == FORM_SUM ? "LIBCONV_SUM3D_VARIANT" : "LIBCONV_SEP3D_VARIANT", variant_count(form));
     if (v >= 0) return v;
-    // defaults: the fastest variants at 256^3 on MI355X (tuner sweep); the
-    // two-plane forms (16 and up) take planes of at most kPlaneMax bytes
+    // defaults: the fastest variants at 256^3 on MI355X, timed inside the
+    // MagicFilter -> Kinetic chain (pipelined MF with at most two blocks per
+    // CU; two-plane K); the plane-buffer forms (16 and up) take planes of at
+    // most kPlaneMax bytes
     const bool fit = plane_fits(b, prec);
     if (form == FORM_SEP) return !fit ? 12 : 45;
     if (!fit) return prec == 4 ? 7 : 8;
