@@ -40,6 +40,8 @@ struct SeppGeo {
   static constexpr int NWX = NW / 2, NWY = NW - NW / 2;
   static constexpr int XPW = (NXT + NWX - 1) / NWX;   // x-tasks per x-wave
   static constexpr int YPW = (NYT + NWY - 1) / NWY;   // y-tasks per y-wave
+  // x-tasks laid out one column group per half-wave when they fit so
+  static constexpr bool HALF = TH <= 32 && 2 * NWX >= TX / QX;
   static constexpr size_t SA = lds_align(sizeof(R) * (TH * TWP + 1));  // one half, spare slot
   static constexpr size_t SB = lds_align(sizeof(R) * TH * BW);
   static constexpr size_t SP = lds_align(sizeof(R) * TY * BW);
@@ -97,11 +99,22 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
   int xo = -1, bo = -1;  // x-task: row start in A, output start in B
   int yo = -1, po = -1;  // y-task: column start in B, output start in P
   if (wv < G::NWX) {
-    const int task = wv * G::XPW + ln;
-    if (ln < G::XPW && task < G::NXT) {
-      const int r = task % TH, c0 = (task / TH) * QX;  // consecutive lanes: consecutive rows
-      xo = r * TWP + c0;
-      bo = r * BW + c0;
+    if constexpr (G::HALF) {
+      // one column group per half-wave (the LDS serves 32 lanes of a b64
+      // access at a time): rows at the odd strides TWP and BW then hit
+      // distinct banks, where two groups in one half-wave collide
+      const int r = ln & 31, grp = 2 * wv + (ln >> 5);
+      if (r < TH && grp < TX / QX) {
+        xo = r * TWP + grp * QX;
+        bo = r * BW + grp * QX;
+      }
+    } else {
+      const int task = wv * G::XPW + ln;
+      if (ln < G::XPW && task < G::NXT) {
+        const int r = task % TH, c0 = (task / TH) * QX;  // consecutive lanes: consecutive rows
+        xo = r * TWP + c0;
+        bo = r * BW + c0;
+      }
     }
   } else {
     const int task = (wv - G::NWX) * G::YPW + ln;
