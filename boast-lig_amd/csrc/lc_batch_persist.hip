@@ -12,7 +12,9 @@
 // i writes what j reads or writes, or j writes what i reads.  A unit of j
 // waits until done[i] equals i's block count (thread 0 spins, bounded: past
 // the limit it records an error in the status word and goes on, so every
-// wave ends).  Publication: every wave waits for its own stores
+// wave ends).  Publication (only for items another item waits for, and
+// Kinetic items whose last block sums the energies; the fence writes back
+// the XCD's L2, which costs): every wave waits for its own stores
 // (s_waitcnt), the block synchronises, then one agent-scope fence (L2 write
 // back) precedes the increment of done[item]; the waiting side fences
 // (L2 invalidate) after its wait.  The block that completes a Kinetic item
@@ -38,6 +40,7 @@ struct PItem {
   int body;
   int nblocks;
   int dep_off, dep_cnt;
+  int publish;   // a later item waits for it, or its last block sums energies
   double* dot;   // energies of a Kinetic item (3 doubles) or null
 };
 
@@ -117,7 +120,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
       case PB_DWT: LC_P_WAV(R, Dwt<Sym8Dec>)(p, bx, by, zc, 0, lds); break;
       case PB_IDWT: LC_P_WAV(R, Idwt<Sym8Rec>)(p, bx, by, zc, 0, lds); break;
     }
-    // publish: this wave's stores complete, then the block, then one fence
+    // publish: this wave's stores complete, then the block, then one fence.
+    // The fence writes back the XCD's L2, so units nobody waits for skip it
+    // (their stores are complete when the kernel ends).
+    if (!pi.publish) continue;  // uniform
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (t == 0) {
@@ -233,6 +239,7 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
       pi.body = PB_SUM_D2;
       tab[i] = conv_args<R, FD2, 16, 16>(it[i], true);
       pi.dot = it[i].dot;
+      pi.publish = pi.dot != nullptr;  // its last block sums the energies
     } else if ((k == BK_DWT || k == BK_IDWT) && id == WAV_SYM8 && wav_plane_fits(it[i].b, sizeof(R))) {
       pi.body = k == BK_DWT ? PB_DWT : PB_IDWT;
       tab[i] = k == BK_DWT ? wav_args_p<R, Dwt<Sym8Dec>>(it[i]) : wav_args_p<R, Idwt<Sym8Rec>>(it[i]);
@@ -253,6 +260,7 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
       if (overlap(rout[i], rin[j]) || overlap(rout[i], rout[j]) || overlap(rin[i], rout[j])) {
         deps.push_back(i);
         depth[j] = std::max(depth[j], depth[i] + 1);
+        items[i].publish = 1;
       }
     items[j].dep_cnt = (int)deps.size() - items[j].dep_off;
   }
