@@ -42,8 +42,13 @@ struct SeppGeo {
   static constexpr int YPW = (NYT + NWY - 1) / NWY;   // y-tasks per y-wave
   // x-tasks laid out one column group per half-wave when they fit so
   static constexpr bool HALF = TH <= 32 && 2 * NWX >= TX / QX;
-  static constexpr size_t SA = lds_align(sizeof(R) * (TH * TWP + 1));  // one half, spare slot
-  static constexpr size_t SB = lds_align(sizeof(R) * TH * BW);
+  // every x- and y-lane has a task: the passes need no lane masks (the
+  // x-lanes past the TH rows filter a spare row 31 nobody reads)
+  static constexpr bool XFULL = HALF && TX / QX == 2 * NWX;
+  static constexpr bool YFULL = NYT == 64 * NWY;
+  static constexpr int AR = XFULL ? 32 : TH;          // rows of A and B
+  static constexpr size_t SA = lds_align(sizeof(R) * (AR * TWP + 1));  // one half, spare slot
+  static constexpr size_t SB = lds_align(sizeof(R) * AR * BW);
   static constexpr size_t SP = lds_align(sizeof(R) * TY * BW);
   static constexpr size_t lds_bytes() { return 2 * (SA + SB + SP); }
   static_assert(TX % QX == 0 && TY % QY == 0 && NT % 128 == 0, "tile shape");
@@ -59,8 +64,8 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
   constexpr int W = G::W, NT = G::NT, TWD = G::TWD, TH = G::TH, TWP = G::TWP, BW = G::BW;
   constexpr int NA = G::NA, NAP = G::NAP;
   constexpr int HA = G::SA / sizeof(R), HB = G::SB / sizeof(R), HP = G::SP / sizeof(R);
-  R* const A = reinterpret_cast<R*>(lds);                    // [2][TH][TWP]
-  R* const B = reinterpret_cast<R*>(lds + 2 * G::SA);        // [2][TH][BW]
+  R* const A = reinterpret_cast<R*>(lds);                    // [2][AR][TWP]
+  R* const B = reinterpret_cast<R*>(lds + 2 * G::SA);        // [2][AR][BW]
   R* const P = reinterpret_cast<R*>(lds + 2 * (G::SA + G::SB));  // [2][TY][BW]
 
   const int t = threadIdx.x;
@@ -95,7 +100,8 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
   };
 
   // task of this lane: the wave's half decides the pass (uniform per wave)
-  const int wv = t >> 6, ln = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6), ln = t & 63;
+  const bool xwave = wv < G::NWX;
   int xo = -1, bo = -1;  // x-task: row start in A, output start in B
   int yo = -1, po = -1;  // y-task: column start in B, output start in P
   if (wv < G::NWX) {
@@ -104,7 +110,7 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
       // access at a time): rows at the odd strides TWP and BW then hit
       // distinct banks, where two groups in one half-wave collide
       const int r = ln & 31, grp = 2 * wv + (ln >> 5);
-      if (r < TH && grp < TX / QX) {
+      if (G::XFULL || (r < TH && grp < TX / QX)) {
         xo = r * TWP + grp * QX;
         bo = r * BW + grp * QX;
       }
@@ -163,8 +169,8 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
         for (int j = 0; j < NAP; ++j) bq[j] = plane_ld<R>(rs, aoff[j]);
         zc_ld.next();
       }
-      // ---- x-task on plane s
-      if (xo >= 0) {
+      // ---- x-task on plane s (x-waves) or y-task on plane s - 1 (y-waves)
+      if (xwave && (G::XFULL || xo >= 0)) {
         const R* row = A + (u & 1) * HA + xo;
         R acc[QX];
 #pragma unroll
@@ -180,8 +186,7 @@ __device__ __forceinline__ void sepp_body(const FusedArgs<R>& p, int bx, int by,
 #pragma unroll
         for (int i = 0; i < QX; ++i) bd[i] = acc[i];
       }
-      // ---- y-task on plane s - 1
-      if (yo >= 0) {
+      if (!xwave && (G::YFULL || yo >= 0)) {
         const R* colp = B + ((u + 1) & 1) * HB + yo;
         R acc[QY];
 #pragma unroll
