@@ -493,6 +493,10 @@ def main():
             y1s, y2s, ds = oracle_chain(O, w, r32['x'])
             out['fp32']['errors'] = {'y1': errs(r32['y1'], y1s), 'y2': errs(r32['y2'], y2s),
                                      'ekin_rel': float(abs(r32['ekin'].sum() - ds.sum()) / abs(ds.sum()))}
+            # and against the oracle on the FP64 input: adds the input's quantisation
+            out['fp32']['errors_vs_fp64_input'] = {
+                'y1': errs(r32['y1'], y1r), 'y2': errs(r32['y2'], y2r),
+                'ekin_rel': float(abs(r32['ekin'].sum() - dr.sum()) / abs(dr.sum()))}
     if not args.no_cpu:
         import oracle as O
         out['cpu_baseline'] = cpu_baseline(O, np, w, r64['x'])

@@ -54,19 +54,36 @@ int hip_fail(hipError_t e, const char* where) {
   } while (0)
 
 // ---------------------------------------------------------------- pools
+// Device scratch and staging buffers, owned per host thread and keyed by
+// (device, stream): work queued asynchronously on one stream never shares a
+// buffer with work on another stream, so stream-API calls on different
+// streams of one thread may overlap.  Calls on one stream are ordered by the
+// stream itself.
 struct Pool {
   void* p = nullptr;
   size_t bytes = 0;
 };
 constexpr int kMaxDev = 16;
 constexpr int kSlots = 6;  // 0 scratch, 1 x staging, 2 y staging, 3 dot, 4-5 extra outputs
-thread_local Pool g_pool[kMaxDev][kSlots];
+struct StreamPools {
+  int dev;
+  hipStream_t st;
+  Pool slot[kSlots];
+};
+thread_local std::vector<StreamPools> g_pools;
 
-int pool_get(int slot, size_t bytes, void** out) {
+int pool_get(int slot, size_t bytes, void** out, hipStream_t st) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDev) return fail(LC_ERR_UNSUPPORTED, "device %d out of range", dev);
-  Pool& pl = g_pool[dev][slot];
+  StreamPools* sp = nullptr;
+  for (auto& e : g_pools)
+    if (e.dev == dev && e.st == st) { sp = &e; break; }
+  if (!sp) {
+    g_pools.push_back(StreamPools{dev, st, {}});
+    sp = &g_pools.back();
+  }
+  Pool& pl = sp->slot[slot];
   if (bytes == 0) bytes = 256;
   if (pl.bytes < bytes) {
     if (pl.p) {
@@ -117,7 +134,7 @@ int stage_in(const R* p, size_t count, int slot, bool copy, hipStream_t st, Arg<
     return LC_OK;
   }
   void* d;
-  int rc = pool_get(slot, count * sizeof(R), &d);
+  int rc = pool_get(slot, count * sizeof(R), &d, st);
   if (rc) return rc;
   a->dev = static_cast<R*>(d);
   a->host_in = p;
@@ -236,7 +253,7 @@ int run_1d(int kind, int id, int d, int idim, const int* n, int bc, const int* n
   if (want_dot) {
     const long np = sweep_partials_needed(p, W);
     void* sc;
-    rc = pool_get(0, sizeof(double) * (size_t)(np + 8), &sc);
+    rc = pool_get(0, sizeof(double) * (size_t)(np + 8), &sc, st);
     if (rc) return rc;
     p.dot_part = static_cast<double*>(sc);
     if (!dot_dev) dot_dev = static_cast<double*>(sc) + np;
@@ -309,7 +326,7 @@ int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, 
     const size_t sb = sep3d_scratch_bytes<R>(b, narr);
     void* sc = nullptr;
     if (getenv("LIBCONV_DIRECT")) {
-      rc = pool_get(0, sb, &sc);
+      rc = pool_get(0, sb, &sc, st);
       if (rc) return rc;
     }
     hipError_t e = sep3d<R>(id, b, narr, ax.dev, ay.dev, (R)a, (R)a_y, st, static_cast<double*>(sc), sb);
@@ -354,7 +371,7 @@ int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, 
   R* w[2] = {nullptr, nullptr};
   if (d > 1) {
     void* sc;
-    rc = pool_get(0, 2 * wmax * (size_t)std::max(narr, 1) * sizeof(R), &sc);
+    rc = pool_get(0, 2 * wmax * (size_t)std::max(narr, 1) * sizeof(R), &sc, st);
     if (rc) return rc;
     w[0] = static_cast<R*>(sc);
     w[1] = w[0] + wmax * (size_t)std::max(narr, 1);
@@ -447,7 +464,7 @@ int run_1ds(int fil, int d, const int* n, const int* bc, const int* nx, const in
   if (want_dot) {
     sbytes = sum3d_scratch_bytes<R>(b, narr);
     void* sc;
-    rc = pool_get(0, sbytes + 8 * sizeof(double), &sc);
+    rc = pool_get(0, sbytes + 8 * sizeof(double), &sc, st);
     if (rc) return rc;
     scratch = static_cast<double*>(sc);
     dot3 = reinterpret_cast<double*>(static_cast<char*>(sc) + sbytes);

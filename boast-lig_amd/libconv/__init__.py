@@ -274,6 +274,50 @@ def _prec(t):
     return 4 if _is_f32(t) else 8
 
 
+def _count(ext, mul=1):
+    c = 1
+    for v in ext:
+        c *= mul * int(v)
+    return c
+
+
+def _dev(t, name, prec, count, optional=False):
+    """Checks a device-tensor argument before its address reaches the C ABI:
+    a torch tensor of the plan's precision (float64 for dot outputs: prec
+    'dot'), contiguous, with at least `count` elements (_resident checks
+    that it is on the GPU).  A wrong
+    dtype or a short or strided buffer would otherwise be read and written
+    with the layout the extents describe."""
+    if t is None:
+        if optional:
+            return
+        raise LibconvError('%s: missing' % name)
+    if not (hasattr(t, 'data_ptr') and hasattr(t, 'is_cuda') and hasattr(t, 'numel')):
+        raise TypeError('%s must be a torch tensor on the GPU' % name)
+    want = 'torch.float64' if prec in (8, 'dot') else 'torch.float32'
+    if str(t.dtype) != want:
+        raise LibconvError('%s: dtype %s, expected %s' % (name, t.dtype, want))
+    if not t.is_contiguous():
+        raise LibconvError('%s: not contiguous' % name)
+    if t.numel() < count:
+        raise LibconvError('%s: %d elements, the extents need %d' % (name, t.numel(), count))
+
+
+def _resident(**ts):
+    """Device residence, checked after every tensor's dtype, layout and size."""
+    for name, t in ts.items():
+        if t is not None and not t.is_cuda:
+            raise LibconvError('%s: not in device memory' % name)
+
+
+def _ext1(n, nx, ny, idim):
+    """Input and output extents of a 1D sweep along idim (the other dims
+    take nx for both, GenericConvolution2.rb:1701-1766)."""
+    out = [int(v) for v in nx]
+    out[idim] = int(ny[idim])
+    return [int(v) for v in nx], out
+
+
 def _rc(rc):
     if rc:
         check()
@@ -283,6 +327,12 @@ def _rc(rc):
 def conv_1d(fil, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0, dot=None, stream=None):
     """Device-pointer sweep (lc_conv_1d); dot: a float64 device tensor or None."""
     n_, nx_, ny_ = _ints(n), _ints(nx), _ints(ny)
+    pr = _prec(x)
+    ei, eo = _ext1(n_, nx_, ny_, idim) if 0 <= idim < len(n_) else (nx_, ny_)
+    _dev(x, 'x', pr, _count(ei) * narr)
+    _dev(y, 'y', pr, _count(eo) * narr)
+    _dev(dot, 'dot', 'dot', 1, optional=True)
+    _resident(x=x, y=y, dot=dot)
     _rc(lib().lc_conv_1d(_prec(x), fil, len(n_), idim, n_.ctypes.data_as(_IP), bc,
                          nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr, _ptr(x), _ptr(y),
                          a, a_x, a_y, _ptr(dot), _stream(stream)))
@@ -290,6 +340,11 @@ def conv_1d(fil, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0, dot=N
 
 def wavelet_1d(wav, inverse, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0, stream=None):
     n_, nx_, ny_ = _ints(n), _ints(nx), _ints(ny)
+    pr = _prec(x)
+    ei, eo = _ext1(n_, nx_, ny_, idim) if 0 <= idim < len(n_) else (nx_, ny_)
+    _dev(x, 'x', pr, _count(ei, 2) * narr)
+    _dev(y, 'y', pr, _count(eo, 2) * narr)
+    _resident(x=x, y=y)
     _rc(lib().lc_wavelet_1d(_prec(x), wav, int(bool(inverse)), len(n_), idim, n_.ctypes.data_as(_IP),
                             bc, nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr, _ptr(x),
                             _ptr(y), a, a_y, _stream(stream)))
@@ -297,6 +352,10 @@ def wavelet_1d(wav, inverse, idim, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0, st
 
 def conv_md(fil, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0, stream=None):
     n_, b_, nx_, ny_ = _ints(n), _ints(bc), _ints(nx), _ints(ny)
+    pr = _prec(x)
+    _dev(x, 'x', pr, _count(nx_) * narr)
+    _dev(y, 'y', pr, _count(ny_) * narr)
+    _resident(x=x, y=y)
     _rc(lib().lc_conv_md(_prec(x), fil, len(n_), n_.ctypes.data_as(_IP), b_.ctypes.data_as(_IP),
                          nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr, _ptr(x), _ptr(y),
                          a, a_x, a_y, _stream(stream)))
@@ -304,6 +363,11 @@ def conv_md(fil, n, bc, nx, ny, narr, x, y, a=1.0, a_x=0.0, a_y=0.0, stream=None
 
 def conv_1ds(fil, n, bc, nx, ny, narr, a, a_x, a_y, x, y, dot=None, stream=None):
     n_, b_, nx_, ny_ = _ints(n), _ints(bc), _ints(nx), _ints(ny)
+    pr = _prec(x)
+    _dev(x, 'x', pr, _count(nx_) * narr)
+    _dev(y, 'y', pr, _count(ny_) * narr)
+    _dev(dot, 'dot', 'dot', len(n_), optional=True)
+    _resident(x=x, y=y, dot=dot)
     av = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
     _rc(lib().lc_conv_1ds(_prec(x), fil, len(n_), n_.ctypes.data_as(_IP), b_.ctypes.data_as(_IP),
                           nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr,
@@ -313,6 +377,10 @@ def conv_1ds(fil, n, bc, nx, ny, narr, a, a_x, a_y, x, y, dot=None, stream=None)
 
 def wavelet_md(wav, inverse, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0, stream=None):
     n_, b_, nx_, ny_ = _ints(n), _ints(bc), _ints(nx), _ints(ny)
+    pr = _prec(x)
+    _dev(x, 'x', pr, _count(nx_, 2) * narr)
+    _dev(y, 'y', pr, _count(ny_, 2) * narr)
+    _resident(x=x, y=y)
     _rc(lib().lc_wavelet_md(_prec(x), wav, int(bool(inverse)), len(n_), n_.ctypes.data_as(_IP),
                             b_.ctypes.data_as(_IP), nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP),
                             narr, _ptr(x), _ptr(y), a, a_y, _stream(stream)))
@@ -332,6 +400,16 @@ def local_hamiltonian(n, a, psig, hpsig, pot, rho, work, ekin=None, stream=None)
     DWT o (IMF o pot o MF + sum_d a_d D2_d) o IDWT psig, rho += psir^2."""
     n_ = _ints(n)
     av = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    pr = _prec(psig)
+    ng = _count([2 * v for v in n_])
+    nr = _count([2 * v + 29 for v in n_])
+    _dev(psig, 'psig', pr, ng)
+    _dev(hpsig, 'hpsig', pr, ng)
+    _dev(pot, 'pot', pr, nr)
+    _dev(rho, 'rho', pr, nr, optional=True)
+    _dev(work, 'work', pr, local_hamiltonian_work(n_))
+    _dev(ekin, 'ekin', 'dot', 3, optional=True)
+    _resident(psig=psig, hpsig=hpsig, pot=pot, rho=rho, work=work, ekin=ekin)
     _rc(lib().lc_local_hamiltonian(_prec(psig), n_.ctypes.data_as(_IP), av.ctypes.data_as(_DP), _ptr(psig),
                                    _ptr(hpsig), _ptr(pot), _ptr(rho), _ptr(work), _ptr(ekin),
                                    _stream(stream)))
@@ -343,6 +421,10 @@ NPERIODIC = -2
 
 def poisson_1d(nord, idim, n, bc, u, du, a=1.0, a_y=0.0, stream=None):
     """lc_poisson_1d (Poisson_broker) on device tensors: du = a D_idim u (+ a_y du)."""
+    pr = _prec(u)
+    _dev(u, 'u', pr, _count(n))
+    _dev(du, 'du', pr, _count(n))
+    _resident(u=u, du=du)
     _rc(lib().lc_poisson_1d(_prec(u), nord, idim, _ints(n).ctypes.data_as(_IP), bc, _ptr(u), _ptr(du),
                             _D(a), _D(a_y), _stream(stream)))
 
@@ -353,18 +435,32 @@ def _h(h):
 
 def div_u_i(geocode, n, u, du, nord, h, cc=None, stream=None):
     hv = _h(h)
+    pr = _prec(u)
+    _dev(u, 'u', pr, 3 * _count(n))
+    _dev(du, 'du', pr, _count(n))
+    _dev(cc, 'cc', pr, _count(n), optional=True)
+    _resident(u=u, du=du, cc=cc)
     _rc(lib().lc_div_u_i(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(du), nord,
                          hv.ctypes.data_as(_DP), _ptr(cc), _stream(stream)))
 
 
 def nabla_u(geocode, n, u, du, nord, h, stream=None):
     hv = _h(h)
+    pr = _prec(u)
+    _dev(u, 'u', pr, _count(n))
+    _dev(du, 'du', pr, 3 * _count(n))
+    _resident(u=u, du=du)
     _rc(lib().lc_nabla_u(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(du), nord,
                          hv.ctypes.data_as(_DP), _stream(stream)))
 
 
 def nabla_u_and_square(geocode, n, u, du, du2, nord, h, stream=None):
     hv = _h(h)
+    pr = _prec(u)
+    _dev(u, 'u', pr, _count(n))
+    _dev(du, 'du', pr, 3 * _count(n))
+    _dev(du2, 'du2', pr, _count(n))
+    _resident(u=u, du=du, du2=du2)
     _rc(lib().lc_nabla_u_and_square(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(du),
                                     _ptr(du2), nord, hv.ctypes.data_as(_DP), _stream(stream)))
 
@@ -421,14 +517,30 @@ class Batch:
     items: dicts with keys kind (BATCH_*), op (FIL_* / WAV_*), n, bc, nx, ny
     (3 ints each, bc may be one int), x, y (device tensors), and optionally
     a (3 floats or one), a_x, a_y, dot (float64 device tensor of 3).  The
-    tensors must stay alive as long as the plan.
+    tensors must stay alive as long as the plan.  The precision follows the
+    tensors' dtype (float64: 8, float32: 4); every x and y must share it,
+    and `prec`, when given, must agree with it.
     """
 
-    def __init__(self, items, prec=8):
+    def __init__(self, items, prec=None):
         L = _batch_lib()
         arr = (_BatchItem * max(len(items), 1))()
         self._keep = []
+        if items:
+            pr = _prec(items[0]['x'])
+            if prec is not None and int(prec) != pr:
+                raise LibconvError('Batch: prec %s does not match the tensors (%s)'
+                                   % (prec, items[0]['x'].dtype))
+            prec = pr
+        elif prec is None:
+            prec = 8
         for k, it in enumerate(items):
+            mul = 2 if int(it['kind']) in (BATCH_DWT, BATCH_IDWT) else 1
+            _dev(it['x'], 'item %d x' % k, prec, _count(it['nx'], mul))
+            _dev(it['y'], 'item %d y' % k, prec, _count(it['ny'], mul))
+            _dev(it.get('dot'), 'item %d dot' % k, 'dot', 3, optional=True)
+        for k, it in enumerate(items):
+            _resident(**{'item %d x' % k: it['x'], 'item %d y' % k: it['y'], 'item %d dot' % k: it.get('dot')})
             b = arr[k]
             b.kind, b.op = int(it['kind']), int(it['op'])
             bc = it['bc']

@@ -199,7 +199,10 @@ void s0s0_sym8_mi_d_(const int *d, const int *n, const int *bc, const int *nx, c
  * NULL = the default stream), int return (LC_OK or an error code).  prec is
  * 8 (FP64) or 4 (FP32 storage and arithmetic); scalars are given in double
  * and rounded to the storage precision; dot outputs are device pointers to
- * FP64 (d values for lc_conv_1ds), written asynchronously.
+ * FP64 (d values for lc_conv_1ds), written asynchronously.  The library's
+ * internal device scratch (energy partials, sweep intermediates) is kept
+ * per host thread and per (device, stream), so calls on different streams
+ * may run concurrently; calls on one stream are ordered by that stream.
  * fil: 0 magic filter (sym8_md), 1 inverse magic filter (sym8_imd),
  *      2 second derivative (sym8_d2), 3 first derivative (sym8_d1);
  * wav: 0 sym8, 1 symicomb8 (magic filter combined with the wavelet).       */

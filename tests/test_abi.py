@@ -85,3 +85,54 @@ def test_stream_api_rejects_bad_precision(lib):
     with pytest.raises(libconv.LibconvError):
         libconv._rc(libconv.lib().lc_conv_1d(3, 0, 1, 0, (ctypes.c_int * 1)(8), 0, (ctypes.c_int * 1)(8),
                                              (ctypes.c_int * 1)(8), 1, None, None, 1.0, 0.0, 0.0, None, None))
+
+
+@pytest.mark.parametrize('case', ['dtype', 'strided', 'short', 'host', 'dot_dtype', 'dot_short'])
+def test_stream_api_checks_tensors_before_the_call(lib, case):
+    """The Python stream API checks dtype, contiguity, size against the
+    extents, device residence and the FP64 energy buffer before any address
+    reaches the C ABI (CPU tensors here: every case must be refused before
+    the library is asked to compute)."""
+    torch = pytest.importorskip('torch')
+    import libconv
+    n = [8, 9, 10]
+    x = torch.zeros(720, dtype=torch.float64)
+    y = torch.zeros(720, dtype=torch.float64)
+    dot = torch.zeros(3, dtype=torch.float64)
+    if case == 'dtype':
+        y = y.float()
+    elif case == 'strided':
+        y = torch.zeros(1440, dtype=torch.float64)[::2]
+    elif case == 'short':
+        y = y[:719]
+    elif case == 'dot_dtype':
+        dot = dot.float()
+    elif case == 'dot_short':
+        dot = dot[:2]
+    msg = {'dtype': 'dtype', 'strided': 'contiguous', 'short': 'elements', 'host': 'device',
+           'dot_dtype': 'dot: dtype', 'dot_short': 'dot: 2 elements'}[case]
+    with pytest.raises(libconv.LibconvError, match=msg):
+        libconv.conv_1ds(2, n, [0] * 3, n, n, 1, [-1.0] * 3, 0.0, 0.0, x, y, dot)
+    if case in ('dtype', 'strided', 'short', 'host'):
+        with pytest.raises(libconv.LibconvError):
+            libconv.conv_md(0, n, [0] * 3, n, n, 1, x, y)
+        with pytest.raises(libconv.LibconvError):
+            libconv.conv_1d(0, 0, n, 0, n, n, 1, x, y)
+
+
+def test_batch_precision_follows_the_tensors(lib):
+    """Batch infers FP64 / FP32 from x and refuses a prec that disagrees or
+    mixed dtypes, before creating a plan (CPU tensors: refused either way)."""
+    torch = pytest.importorskip('torch')
+    import libconv
+    n = [10, 10, 10]
+    xf = torch.zeros(1000, dtype=torch.float32)
+    item = dict(kind=0, op=0, n=n, bc=0, nx=n, ny=n, x=xf, y=torch.zeros(1000, dtype=torch.float32))
+    with pytest.raises(libconv.LibconvError, match='prec'):
+        libconv.Batch([item], prec=8)
+    with pytest.raises(libconv.LibconvError, match='dtype'):
+        libconv.Batch([dict(item, y=torch.zeros(1000, dtype=torch.float64))])
+    with pytest.raises(libconv.LibconvError, match='elements'):
+        libconv.Batch([dict(item, y=torch.zeros(999, dtype=torch.float32))])
+    with pytest.raises(libconv.LibconvError, match='device'):
+        libconv.Batch([item])

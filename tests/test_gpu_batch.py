@@ -1,7 +1,8 @@
 """GPU parity of the single-pass 3D wavelet transform and of batch plans.
 
 The single-pass 3D DWT/IDWT is checked against the FP64 oracle's three-sweep
-composition (1e-13 relative max-norm; FP32 within 4e-6).  A batch plan runs
+composition (1e-13 relative max-norm; FP32 within 1e-6 on the unrounded
+input; measured 4.1e-7).  A batch plan runs
 the same kernels over many items from a device argument table: its outputs
 must equal, bit for bit, the outputs of one call per item (the z chunking of
 a launch does not change any output's arithmetic); its FP64 energies sum the
@@ -69,7 +70,7 @@ def test_wavelet_3d_single_pass(lc, gpu, wav, bcs, inverse):
     y32 = torch.from_numpy(y0.astype(np.float32)).cuda()
     lc.wavelet_md(wav, inverse, n, bc_call, nx, ny, narr, xd.float(), y32, a=a, a_y=a_y)
     torch.cuda.synchronize()
-    assert rel(y32.cpu().numpy(), ref) < 4e-6
+    assert rel(y32.cpu().numpy(), ref) < 1e-6
 
 
 @pytest.mark.skip(reason='the single-pass symicomb8 kernel gave wrong high bands and then faulted '

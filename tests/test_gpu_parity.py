@@ -2,10 +2,11 @@
 
 Bar: FP64 within 1e-13 relative max-norm of the oracle (the path's stated
 tolerance; the reference's own tests use 1e-14 for MagicFilter, 1e-12 for
-Kinetic, 1e-11 for wavelets); FP32 storage and arithmetic within 2e-6
-(MagicFilter / wavelets) and 1e-5 (Kinetic, whose sum of |taps| is 9.9) of
-the FP64 oracle, relative to the oracle's max-norm; FP64 energies within
-1e-12 relative.  Inputs are seeded U[-1, 1].
+Kinetic, 1e-11 for wavelets); FP32 storage and arithmetic within 1e-6 of the
+FP64 oracle on the unrounded FP64 input (so input quantisation included),
+relative to the oracle's max-norm (measured on MI355X: <= 3.7e-7 MagicFilter,
+1.5e-7 Kinetic, 2.7e-7 local Hamiltonian); FP64 energies within 1e-12
+relative, energies over FP32 data within 1e-6.  Inputs are seeded U[-1, 1].
 """
 import numpy as np
 import pytest
@@ -15,6 +16,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-13
+TOL32 = 1e-6
 
 
 def rel(a, b):
@@ -315,16 +317,16 @@ def test_fp32_forms_against_fp64_oracle(lc):
     ref = np.zeros(int(np.prod(n)))
     lc.MagicFilter(fp32=True).run(3, n, [0, 0, 0], n, n, 1, x.astype(np.float32), y, None, None)
     O.s0s0_md(O.MD, n, [0, 0, 0], n, n, 1, x, ref)
-    assert rel(y, ref) < 2e-6
+    assert rel(y, ref) < TOL32
     a = [-5.5, -4.0, -3.0]
     yk = np.zeros(int(np.prod(n)), np.float32)
     st = lc.Kinetic(fp32=True, dot=True).run(3, n, [0, 0, 0], x.astype(np.float32), yk, None, None,
                                             a, 0.0, 0.0)
     refk = np.zeros(int(np.prod(n)))
     rdot = O.s0s0_1ds(O.D2, n, [0, 0, 0], n, n, 1, a, 0.0, 0.0, x, refk)
-    assert rel(yk, refk) < 1e-5
+    assert rel(yk, refk) < TOL32
     # FP64 energies over FP32 data: the error is the data's rounding only
-    assert np.abs(st['reference_return']['dot_in'] - rdot).max() <= 1e-6 * np.abs(rdot).max()
+    assert np.abs(st['reference_return']['dot_in'] - rdot).max() <= TOL32 * np.abs(rdot).max()
 
 
 # ------------------------------------------------------------------ stream API
@@ -352,6 +354,42 @@ def test_stream_api_device_tensors(lc, gpu):
     assert rel(y1.cpu().numpy(), r1) < TOL64
     assert rel(y2.cpu().numpy(), r2) < TOL64
     assert np.abs(dot.cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
+
+
+def test_stream_api_two_streams_keep_their_scratch(lc, gpu):
+    """Energy partials and sweep intermediates live in per-(device, stream)
+    scratch: calls of one host thread on two streams, queued without a
+    synchronisation between them, give the bits of the same calls run one
+    after the other on one stream."""
+    torch = gpu
+    rng = np.random.default_rng(710)
+    n = [96, 80, 88]
+    a = [-0.5 / 0.3 ** 2, -0.5 / 0.25 ** 2, -0.5 / 0.2 ** 2]
+    xs = [torch.from_numpy(rng.uniform(-1, 1, int(np.prod(n)))).cuda() for _ in range(2)]
+    nsw = [n[0] + 28, n[1], n[2]]
+
+    def work(x, s):
+        y = torch.empty_like(x)
+        dot = torch.zeros(3, dtype=torch.float64, device='cuda')
+        w = torch.empty(int(np.prod(nsw)), dtype=torch.float64, device='cuda')
+        d1 = torch.zeros(1, dtype=torch.float64, device='cuda')
+        c = torch.empty_like(x)
+        pairs = [v // 2 for v in n]
+        lc.conv_1ds(2, n, [0] * 3, n, n, 1, a, 0.0, 0.0, x, y, dot=dot, stream=s)   # energy partials
+        lc.conv_1d(2, 0, n, 1, n, nsw, 1, x, w, a=-2.0, dot=d1, stream=s)           # 1D partials
+        lc.wavelet_md(1, False, pairs, [0] * 3, pairs, pairs, 1, x, c, stream=s)    # intermediates
+        return y, dot, w, d1, c
+
+    s0 = torch.cuda.current_stream()
+    ref = [work(x, s0) for x in xs]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    got = [work(xs[0], s1), work(xs[1], s2)]
+    torch.cuda.synchronize()
+    for r, g in zip(ref, got):
+        for u, v in zip(r, g):
+            assert torch.equal(u, v)
 
 
 # ------------------------------------------------------------ local Hamiltonian
@@ -383,10 +421,10 @@ def test_local_hamiltonian_matches_oracle(lc, gpu, dtype_name):
     ekin = torch.zeros(3, dtype=torch.float64, device='cuda')
     lc.local_hamiltonian(n, a, tp, th, tpot, trho, work, ekin)
     torch.cuda.synchronize()
-    tol = TOL64 if dtype_name == 'float64' else 2e-5
+    tol = TOL64 if dtype_name == 'float64' else TOL32
     assert rel(th.cpu().numpy(), ref_h) < tol
     assert rel(trho.cpu().numpy(), ref_rho) < tol
-    assert np.abs(ekin.cpu().numpy() - ref_e).max() <= (1e-12 if dtype_name == 'float64' else 1e-5) * np.abs(ref_e).max()
+    assert np.abs(ekin.cpu().numpy() - ref_e).max() <= (1e-12 if dtype_name == 'float64' else TOL32) * np.abs(ref_e).max()
 
 
 def test_local_hamiltonian_is_symmetric(lc, gpu):

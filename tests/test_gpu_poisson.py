@@ -1,9 +1,8 @@
 """GPU parity of the Poisson nabla operators (BigDFT/Poisson2.rb) against the
 FP64 oracle and the reference's own outputs (tests/golden/poisson.npz).
 
-Bar: FP64 within 1e-13 relative max-norm of the oracle; FP32 within 1e-5 of
-the FP64 oracle on the FP32-rounded input (the nord-16 stencils sum |c| ~ 6 /
-h times the field)."""
+Bar: FP64 within 1e-13 relative max-norm of the oracle; FP32 within 1e-6 of
+the FP64 oracle on the FP32-rounded input (measured on MI355X: <= 3.3e-7)."""
 import os
 
 import numpy as np
@@ -41,7 +40,7 @@ def test_div_and_nabla_match_oracle(lc, gpu, nord, geocode, dtype_name):
     u = rng.uniform(-1, 1, 3 * np_)
     if dtype_name == 'float32':
         u = u.astype(np.float32).astype(np.float64)
-    tol = TOL64 if dtype_name == 'float64' else 1e-5
+    tol = TOL64 if dtype_name == 'float64' else 1e-6
     du, cc = np.zeros(np_), np.zeros(np_)
     O.div_u_i(geocode, n, u, du, nord, h, cc)
     tu = torch.from_numpy(u).to('cuda', dtype)

@@ -400,6 +400,111 @@ def test_free_bc_is_grow_restricted_to_the_input_box():
     assert np.abs(np.asarray(df) - np.asarray(dg)).max() <= 1e-14 * np.abs(dg).max()
 
 
+def _d2_from_decimals():
+    """The 29 second-derivative taps straight from their decimal strings
+    (WaveletFilters.rb:74-103), independent of the oracle's tables."""
+    import importlib.util
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        'boast-lig_amd', 'csrc', 'gen_taps.py')
+    spec = importlib.util.spec_from_file_location('gen_taps', path)
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return np.array([float(s) for s in m.d2_full()])
+
+
+def _kin_ext(n, bc):
+    """Input and output extents of the 29-tap sum form per BC."""
+    return ([v + 28 if b == -1 else v for v, b in zip(n, bc)],
+            [v + 28 if b == 1 else v for v, b in zip(n, bc)])
+
+
+def _kinetic_brute(n, bc, x, a, a_x, a_y, y0):
+    """Brute-force 3D Kinetic sum (BigDFT/Kinetic.rb:1068-1090,
+    GenericConvolution2.rb:2160-2240 with the zero extension of free dims):
+    x~ is built explicitly on the output box widened by 14 on every side --
+    periodic dims wrap, grow/free dims are zero outside [0, n), shrink dims
+    hold the stored input at logical index j (storage origin -14) -- then
+    every output is sum_d a_d sum_l f[l] x~(p + l e_d) + a_x x~(p)
+    + a_y y0(p), and the energies are <x~, a_d D2_d x~> over the output box.
+    Loops over axes and taps only: no code shared with the oracle."""
+    f = _d2_from_decimals()
+    nx, ny = _kin_ext(n, bc)
+    olo = [-14 if b == 1 else 0 for b in bc]
+    oext = ny
+    X = x.reshape(nx, order='F')
+    # logical index of every slot of the widened box, per dim
+    axes = [np.arange(olo[k] - 14, olo[k] + oext[k] + 14) for k in range(3)]
+    maps, masks = [], []
+    for k in range(3):
+        j = axes[k]
+        if bc[k] == 0:
+            maps.append(np.mod(j, n[k]))
+            masks.append(np.ones(j.size, bool))
+        elif bc[k] == -1:
+            pos = j + 14
+            ok = (pos >= 0) & (pos < nx[k])
+            maps.append(np.where(ok, pos, 0))
+            masks.append(ok)
+        else:
+            ok = (j >= 0) & (j < n[k])
+            maps.append(np.where(ok, j, 0))
+            masks.append(ok)
+    Xt = X[np.ix_(maps[0], maps[1], maps[2])]
+    Xt = Xt * (masks[0][:, None, None] & masks[1][None, :, None] & masks[2][None, None, :])
+    inner = tuple(slice(14, 14 + oext[k]) for k in range(3))
+    xc = Xt[inner]
+    y = a_x * xc + (a_y * y0.reshape(ny, order='F') if a_y != 0.0 else 0.0)
+    dots = []
+    for k in range(3):
+        t = np.zeros(oext)
+        for li, l in enumerate(range(-14, 15)):
+            sl = list(inner)
+            sl[k] = slice(14 + l, 14 + l + oext[k])
+            t += f[li] * Xt[tuple(sl)]
+        t *= a[k]
+        dots.append(float((xc * t).sum()))
+        y = y + t
+    return y.reshape(-1, order='F'), np.array(dots)
+
+
+@pytest.mark.parametrize('bc', [(1, 1, 1), (-1, -1, -1), (0, 1, -1), (-1, 0, 1), (0, 0, 0)])
+def test_kinetic_sum_form_brute_force(bc):
+    """The oracle's 3D Kinetic sum form (lco_s0s0_1ds) against the
+    independent brute force above, under grow, shrink and mixed BCs, with
+    a_x, a_y and the three energies; n below the filter width included."""
+    rng = np.random.default_rng(71)
+    a = [-2.3, -0.9, -1.7]
+    for n in ([7, 9, 6], [4, 31, 5]):
+        nx, ny = _kin_ext(n, bc)
+        x = rng.uniform(-1, 1, int(np.prod(nx)))
+        y0 = rng.uniform(-1, 1, int(np.prod(ny)))
+        for a_x, a_y in ((0.0, 0.0), (0.75, 0.5)):
+            ref, rdots = _kinetic_brute(n, list(bc), x, a, a_x, a_y, y0)
+            y = y0.copy()
+            dots = O.s0s0_1ds(O.D2, n, list(bc), nx, ny, 1, a, a_x, a_y, x, y)
+            assert rel(y, ref) < 1e-14, (n, bc, a_x, a_y)
+            assert np.abs(dots - rdots).max() < 1e-13 * np.abs(rdots).max(), (n, bc)
+
+
+def test_kinetic_sum_form_grow_shrink_adjoint():
+    """<K_grow x, v> = <x, K_shrink v>: the grow form zero-extends and
+    convolves with the symmetric D2 taps, the shrink form restricts and
+    correlates with them, and the a_x term embeds / restricts.  Mixed with a
+    periodic dim (self-adjoint)."""
+    rng = np.random.default_rng(72)
+    a = [-1.1, -2.2, -0.6]
+    for n, g, s in (([8, 7, 9], [1, 1, 1], [-1, -1, -1]), ([6, 10, 5], [1, 0, 1], [-1, 0, -1])):
+        ng = [v + 28 if b == 1 else v for v, b in zip(n, g)]
+        x = rng.uniform(-1, 1, int(np.prod(n)))
+        v = rng.uniform(-1, 1, int(np.prod(ng)))
+        kg = np.zeros(v.size)
+        O.s0s0_1ds(O.D2, n, g, n, ng, 1, a, 0.4, 0.0, x, kg)
+        ks = np.zeros(x.size)
+        O.s0s0_1ds(O.D2, n, s, ng, n, 1, a, 0.4, 0.0, v, ks)
+        lhs, rhs = float(kg @ v), float(x @ ks)
+        assert abs(lhs - rhs) < 1e-12 * max(abs(lhs), 1.0), (n, lhs, rhs)
+
+
 def test_local_hamiltonian_chain_is_symmetric():
     """The chain of libconv.f90:238-295 is H = DWT_s (IMF_s V MF_g + K_free)
     IDWT_g.  DWT shrink and IDWT grow are adjoint (orthonormal wavelets), the
