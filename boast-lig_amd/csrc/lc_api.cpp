@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/libconv.h"
@@ -103,6 +104,34 @@ int pool_get(int slot, size_t bytes, void** out, hipStream_t st) {
     pl.bytes = want;
   }
   *out = pl.p;
+  return LC_OK;
+}
+
+// ------------------------------------------------ static scratch
+// Device scratch of lc_conv_1ds_noalloc: a file-scope device array, one copy
+// per GPU, that the library never allocates or frees.  In host code the
+// array's name is only a stand-in for the symbol: its device address comes
+// from hipGetSymbolAddress, taken once per device (with that device current)
+// on the entry point's first call there, and only that address is handed to
+// kernels.  Every call on one GPU shares the array.
+constexpr size_t kStaticScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+void* g_static_scratch[kMaxDev] = {};
+std::mutex g_static_mu;
+
+int static_scratch(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDev) return fail(LC_ERR_UNSUPPORTED, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  if (!g_static_scratch[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(LC_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned a null address");
+    g_static_scratch[dev] = p;
+  }
+  *out = g_static_scratch[dev];
   return LC_OK;
 }
 
@@ -423,10 +452,13 @@ int run_md(int kind, int id, int d, const int* n, const int* bc, const int* nx, 
 }
 
 // kinetic-style sum over d <= 3 dims
+// use_static: the energy partials go to the library's static device array
+// instead of the pool (the allocation-free entry point); shapes whose
+// partials do not fit are refused before the device is touched
 template <typename R>
 int run_1ds(int fil, int d, const int* n, const int* bc, const int* nx, const int* ny, int narr,
             const double* a, double a_x, double a_y, const R* x, R* y, double* dot_host,
-            double* dot_dev_user, hipStream_t st, bool device_api) {
+            double* dot_dev_user, hipStream_t st, bool device_api, bool use_static = false) {
   const char* who = "s0s0_1ds";
   int rc = check_dims(d, n, nx, ny, narr, who);
   if (rc) return rc;
@@ -447,6 +479,18 @@ int run_1ds(int fil, int d, const int* n, const int* bc, const int* nx, const in
     }
   }
   long tin = (long)b.nx[0] * b.nx[1] * b.nx[2], tout = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  const bool want_dot = dot_host || dot_dev_user;
+  const size_t sbytes = want_dot ? sum3d_scratch_bytes<R>(b, narr) : 0;
+  void* fixed = nullptr;
+  if (use_static) {
+    if (want_dot && sbytes + 8 * sizeof(double) > kStaticScratchBytes)
+      return fail(LC_ERR_UNSUPPORTED,
+                  "%s: the energy partials of this shape need %zu bytes of scratch, above the "
+                  "%zu-byte static buffer of the allocation-free entry point",
+                  who, sbytes + 8 * sizeof(double), kStaticScratchBytes);
+    rc = static_scratch(&fixed);
+    if (rc) return rc;
+  }
   Arg<R> ax, ay;
   if (device_api) {
     ax.dev = const_cast<R*>(x);
@@ -457,15 +501,14 @@ int run_1ds(int fil, int d, const int* n, const int* bc, const int* nx, const in
     rc = stage_in<R>(y, (size_t)tout * narr, 2, true, st, &ay);
     if (rc) return rc;
   }
-  const bool want_dot = dot_host || dot_dev_user;
   double* dot3 = nullptr;
   double* scratch = nullptr;
-  size_t sbytes = 0;
   if (want_dot) {
-    sbytes = sum3d_scratch_bytes<R>(b, narr);
-    void* sc;
-    rc = pool_get(0, sbytes + 8 * sizeof(double), &sc, st);
-    if (rc) return rc;
+    void* sc = fixed;
+    if (!sc) {
+      rc = pool_get(0, sbytes + 8 * sizeof(double), &sc, st);
+      if (rc) return rc;
+    }
     scratch = static_cast<double*>(sc);
     dot3 = reinterpret_cast<double*>(static_cast<char*>(sc) + sbytes);
   }
@@ -958,6 +1001,17 @@ int lc_conv_1ds(int prec, int fil, int d, const int* n, const int* bc, const int
     return run_1ds<float>(fil, d, n, bc, nx, ny, narr, a, a_x, a_y, (const float*)x, (float*)y,
                           nullptr, dot_dev, hs(stream), true);
   return fail(LC_ERR_ARG, "lc_conv_1ds: prec %d not in {4, 8}", prec);
+}
+int lc_conv_1ds_noalloc(int prec, int fil, int d, const int* n, const int* bc, const int* nx,
+                        const int* ny, int narr, const double* a, double a_x, double a_y, const void* x,
+                        void* y, double* dot_dev, void* stream) {
+  if (prec == 8)
+    return run_1ds<double>(fil, d, n, bc, nx, ny, narr, a, a_x, a_y, (const double*)x, (double*)y,
+                           nullptr, dot_dev, hs(stream), true, true);
+  if (prec == 4)
+    return run_1ds<float>(fil, d, n, bc, nx, ny, narr, a, a_x, a_y, (const float*)x, (float*)y,
+                          nullptr, dot_dev, hs(stream), true, true);
+  return fail(LC_ERR_ARG, "lc_conv_1ds_noalloc: prec %d not in {4, 8}", prec);
 }
 int lc_wavelet_md(int prec, int wav, int inverse, int d, const int* n, const int* bc,
                   const int* nx, const int* ny, int narr, const void* x, void* y, double a,

@@ -99,7 +99,9 @@ def lib():
         L.lc_conv_md.argtypes = [_I, _I, _I, _IP, _IP, _IP, _IP, _I, _VP, _VP, _D, _D, _D, _VP]
         L.lc_conv_1ds.argtypes = [_I, _I, _I, _IP, _IP, _IP, _IP, _I, _DP, _D, _D, _VP, _VP, _VP, _VP]
         L.lc_wavelet_md.argtypes = [_I, _I, _I, _I, _IP, _IP, _IP, _IP, _I, _VP, _VP, _D, _D, _VP]
-        for n in ('lc_conv_1d', 'lc_wavelet_1d', 'lc_conv_md', 'lc_conv_1ds', 'lc_wavelet_md'):
+        L.lc_conv_1ds_noalloc.argtypes = L.lc_conv_1ds.argtypes
+        for n in ('lc_conv_1d', 'lc_wavelet_1d', 'lc_conv_md', 'lc_conv_1ds', 'lc_wavelet_md',
+                  'lc_conv_1ds_noalloc'):
             getattr(L, n).restype = _I
         _lib = L
     return _lib
@@ -373,6 +375,23 @@ def conv_1ds(fil, n, bc, nx, ny, narr, a, a_x, a_y, x, y, dot=None, stream=None)
                           nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr,
                           av.ctypes.data_as(_DP), a_x, a_y, _ptr(x), _ptr(y), _ptr(dot),
                           _stream(stream)))
+
+
+def conv_1ds_noalloc(fil, n, bc, nx, ny, narr, a, a_x, a_y, x, y, dot=None, stream=None):
+    """conv_1ds through lc_conv_1ds_noalloc: the energy partials use the
+    library's static device buffer, no allocation (calls on one GPU must not
+    overlap)."""
+    n_, b_, nx_, ny_ = _ints(n), _ints(bc), _ints(nx), _ints(ny)
+    av = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    pr = _prec(x)
+    _dev(x, 'x', pr, _count(nx_) * narr)
+    _dev(y, 'y', pr, _count(ny_) * narr)
+    _dev(dot, 'dot', 'dot', len(n_), optional=True)
+    _resident(x=x, y=y, dot=dot)
+    _rc(lib().lc_conv_1ds_noalloc(pr, fil, len(n_), n_.ctypes.data_as(_IP), b_.ctypes.data_as(_IP),
+                                  nx_.ctypes.data_as(_IP), ny_.ctypes.data_as(_IP), narr,
+                                  av.ctypes.data_as(_DP), a_x, a_y, _ptr(x), _ptr(y), _ptr(dot),
+                                  _stream(stream)))
 
 
 def wavelet_md(wav, inverse, n, bc, nx, ny, narr, x, y, a=1.0, a_y=0.0, stream=None):

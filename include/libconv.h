@@ -225,6 +225,20 @@ int lc_conv_1ds(int prec, int fil, int d, const int *n, const int *bc, const int
 int lc_wavelet_md(int prec, int wav, int inverse, int d, const int *n, const int *bc,
                   const int *nx, const int *ny, int narr, const void *x, void *y, double a,
                   double a_y, void *stream);
+/* lc_conv_1ds without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as lc_conv_1ds; the energy partials go to a static 16 MiB device
+ * buffer of the library (one per GPU, held until the process ends) instead
+ * of the pool, so it calls no hipMalloc / hipFree.  Shapes whose partials
+ * may need more (output grids beyond about 440^3 points, with energies) are
+ * refused with LC_ERR_UNSUPPORTED before the device is touched.  All calls on one GPU share that buffer: calls
+ * that overlap on one GPU (other streams, other threads) race, so serialise
+ * them.  The buffer keeps the last call's contents; every call writes all of
+ * what it reads back.                                                        */
+int lc_conv_1ds_noalloc(int prec, int fil, int d, const int *n, const int *bc, const int *nx,
+                        const int *ny, int narr, const double *a, double a_x, double a_y,
+                        const void *x, void *y, double *dot_dev, void *stream);
 /* ------------------------------------------- batched 3D operators
  * No counterpart in the reference, which applies each operator to one
  * orbital at a time (BigDFT/GenericConvolution2.rb:2131-2245 composes one

@@ -136,3 +136,22 @@ def test_batch_precision_follows_the_tensors(lib):
         libconv.Batch([dict(item, y=torch.zeros(999, dtype=torch.float32))])
     with pytest.raises(libconv.LibconvError, match='device'):
         libconv.Batch([item])
+
+
+def test_noalloc_entry_point_refuses_oversized_partials_before_the_device(lib):
+    """lc_conv_1ds_noalloc keeps its energy partials in a static 16 MiB device
+    array: a 480^3 grid with energies needs more and is refused with
+    LC_ERR_UNSUPPORTED before any device call (so this runs without a GPU;
+    the host buffers passed are never touched)."""
+    import libconv
+    L = libconv.lib()
+    n = (ctypes.c_int * 3)(480, 480, 480)
+    bc = (ctypes.c_int * 3)(0, 0, 0)
+    a = (ctypes.c_double * 3)(-1.0, -1.0, -1.0)
+    buf = np.zeros(4)
+    p = ctypes.c_void_p(buf.ctypes.data)
+    rc = L.lc_conv_1ds_noalloc(8, 2, 3, n, bc, n, n, 1, a, 0.0, 0.0, p, p, buf.ctypes.data_as(
+        ctypes.POINTER(ctypes.c_double)), None)
+    assert rc == 2  # LC_ERR_UNSUPPORTED
+    assert L.lc_last_error() == 2
+    assert (buf == 0).all()
