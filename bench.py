@@ -292,6 +292,12 @@ def run_batch(torch, lc, dtype, reps, count=512):
     items, pts, dots = batch_items(torch, lc, orbs, dtype)
     prec = 4 if dtype == torch.float32 else 8
     plan = lc.Batch(items, prec=prec)
+    # the stages' tiles and z chunking from the autotuner (lc_batch_tune,
+    # BigDFT/GenericConvolution2.rb:744-837 in spirit): every variant of every
+    # stage timed on the plan's own items, the fastest kept
+    tuned_ms = plan.tune(repeat=3)
+    variants = [plan.stage_variant(k) for k in range(len(tuned_ms))]
+    byts = sum((it['x'].numel() + it['y'].numel()) * it['x'].element_size() for it in items)
     st = torch.cuda.current_stream()
     a = [-0.5 / H ** 2] * 3
 
@@ -339,6 +345,9 @@ def run_batch(torch, lc, dtype, reps, count=512):
             'persistent_grid': pinfo['grid'], 'persistent_status': pinfo['status'],
             'persistent_outputs_equal_sampled': bool(same_p),
             'Gpts_s_graph': round(pts / (t_graph * 1e-3) / 1e9, 3),
+            'algorithmic_bytes': byts, 'GB_s_graph': round(byts / (t_graph * 1e-3) / 1e9, 1),
+            'frac_hbm_graph': round(byts / (t_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            'tuned_stage_ms': [round(v, 4) for v in tuned_ms], 'tuned_stage_variants': variants,
             'Gpts_s_per_orbital': round(pts / (t_single * 1e-3) / 1e9, 3),
             'outputs_equal_sampled': bool(same), 'ekin_rel_diff': ddiff,
             'sizes': 'per dim U{24..96}, BC U{periodic, grow, shrink}, seed 5; each orbital: '
@@ -376,19 +385,96 @@ def oracle_chain(O, w, x):
 
 
 def cpu_baseline(O, np, w, x):
-    """The oracle's OpenMP chain on the same workload and input: one warm-up,
-    then the min of 5 runs (BigDFT/MagicFilter_test.rb:69-79 keeps the min
-    of repeated runs)."""
-    oracle_chain(O, w, x)
+    """The CPU baseline on the same workload and input: oracle/liblc_cpu.so,
+    the reference's generated sweep structure restated in C++/OpenMP (three
+    MagicFilter sweeps through work arrays, three Kinetic passes with the
+    energies, lines unrolled along the contiguous dim, cache-staged strided
+    lines), all host threads, work arrays allocated before timing.  One
+    warm-up, then the min of 5 runs (BigDFT/MagicFilter_test.rb:69-79 keeps
+    the min of repeated runs)."""
+    mf, kin = chain_args(w)
+    a = [-0.5 / H ** 2] * 3
+    y1 = np.zeros(w['ny1'] ** 3)
+    y2 = np.zeros(w['ny2'] ** 3)
+    w1 = np.zeros(mf['ny'][0] * mf['nx'][1] * mf['nx'][2])
+    w2 = np.zeros(mf['ny'][0] * mf['ny'][1] * mf['nx'][2])
+
+    def chain():
+        O.cpu_magicfilter3d(mf['n'], mf['bc'], mf['nx'], mf['ny'], x, y1, w1, w2)
+        return O.cpu_kinetic3d(kin['n'], kin['bc'], kin['nx'], kin['ny'], a, y1, y2)
+
+    chain()
     best = 1e30
     for _ in range(5):
         t0 = time.perf_counter()
-        oracle_chain(O, w, x)
+        chain()
         best = min(best, time.perf_counter() - t0)
-    return dict(value=w['pts'] / best / 1e9, unit='Gpts/s', cores=O.num_threads(), kind='port',
-                sample='the full workload (%s), FP64, min of 5 after 1 warm-up; '
-                       'oracle/liblc_oracle.so (C restatement, OpenMP)' % w['desc'],
-                seconds=round(best, 4))
+    model = ''
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=w['pts'] / best / 1e9, unit='Gpts/s', cores=O.cpu_threads(), kind='port',
+                sample='the full workload (%s), FP64, min of 5 after 1 warm-up; oracle/liblc_cpu.so '
+                       '(the generated sweep structure in C++/OpenMP, x86-64-v3)' % w['desc'],
+                seconds=round(best, 4), cpu=model, nproc=os.cpu_count(),
+                omp_num_threads=os.environ.get('OMP_NUM_THREADS'))
+
+
+def copy_ceiling(torch, mb=1024, reps=10):
+    """Device copy of mb MiB buffers (rotated past the MALL): read + write
+    bytes per second, the practical ceiling of a streaming kernel."""
+    n = (mb << 20) // 8
+    bufs = [(torch.ones(n, dtype=torch.float64, device='cuda'), torch.empty(n, dtype=torch.float64, device='cuda'))
+            for _ in range(3)]
+    st = torch.cuda.current_stream()
+    for i in range(3):
+        bufs[i][1].copy_(bufs[i][0])
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for i in range(reps):
+            b = bufs[i % 3]
+            b[1].copy_(b[0])
+        e1.record(st)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) / reps)
+    del bufs
+    return round(2 * n * 8 / (best * 1e-3) / 1e9, 1)
+
+
+def run_c1(torch, lc, reps=2000):
+    """MagicFilter sweep along dim 0 of a 32^3 FP64 grid, periodic: kernel
+    time per call (events around back-to-back calls) and the wall time of one
+    call on the host (launch-latency bound)."""
+    import numpy as np
+    n = [32, 32, 32]
+    g = torch.Generator(device='cuda').manual_seed(1)
+    x = torch.rand(32 ** 3, dtype=torch.float64, device='cuda', generator=g) * 2 - 1
+    y = torch.empty_like(x)
+    st = torch.cuda.current_stream()
+    for _ in range(20):
+        lc.conv_1d(lc.FIL_MD, 0, n, 0, n, n, 1, x, y, stream=st)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with quiet_host():
+        t0 = time.perf_counter()
+        e0.record(st)
+        for _ in range(reps):
+            lc.conv_1d(lc.FIL_MD, 0, n, 0, n, n, 1, x, y, stream=st)
+        e1.record(st)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+    dev = e0.elapsed_time(e1) / reps
+    byts = 2 * 32 ** 3 * 8
+    return {'workload': 'MF 1D sweep, dim 0, 32^3, periodic, FP64', 'device_us_per_call': round(dev * 1e3, 2),
+            'host_us_per_call': round(wall * 1e6, 2), 'Gpts_s': round(32 ** 3 / (dev * 1e-3) / 1e9, 3),
+            'GB_s': round(byts / (dev * 1e-3) / 1e9, 1), 'algorithmic_bytes': byts}
 
 
 def main():
@@ -443,6 +529,7 @@ def main():
         kname = lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, kina['n'], kina['bc'], kina['nx'], kina['ny'])
     else:
         kname = lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, mfa['n'], mfa['bc'], mfa['nx'], mfa['ny'])
+    ceiling = copy_ceiling(torch)
     out = {
         'metric': 'Gpts/s & % of HBM roofline, 3D MagicFilter+Kinetic FP64 vs FP32 at 1e-6',
         'value': round(gpts, 4),
@@ -461,9 +548,11 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': measured_traffic(kname),
                      'kernel': kname or dom,
-                     'avg_ms': round(dom_t, 5), 'algorithmic_bytes': dom_b},
+                     'avg_ms': round(dom_t, 5), 'algorithmic_bytes': dom_b,
+                     'copy_ceiling_GBs': ceiling, 'frac_of_copy': round(achieved / ceiling, 4)},
         'chain': {'GB_s': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9, 1),
                   'frac_hbm': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  'frac_of_copy': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9 / ceiling, 4),
                   'algorithmic_bytes': chain_bytes, 'mf_ms': round(r64['t_mf'], 5),
                   'kin_ms': round(r64['t_k'], 5)},
     }
@@ -474,6 +563,22 @@ def main():
                        'chain_GB_s': round(chain_bytes / 2 / (r32['ms'] * 1e-3) / 1e9, 1),
                        'mf_ms': round(r32['t_mf'], 5), 'kin_ms': round(r32['t_k'], 5)}
     if not args.no_extra:
+        out['c1'] = run_c1(torch, lc)
+        for wl in ('c2p', 'c2g'):
+            wc = workload(wl)
+            rc = run_gpu(torch, lc, wc, torch.float64, args.steps, args.warmup)
+            cb = 8 * (wc['nx'] ** 3 + 2 * wc['ny1'] ** 3 + wc['ny2'] ** 3)
+            line = {'workload': wc['desc'], 'ms': round(rc['ms'], 5),
+                    'Gpts_s': round(wc['pts'] / (rc['ms'] * 1e-3) / 1e9, 3),
+                    'GB_s': round(cb / (rc['ms'] * 1e-3) / 1e9, 1),
+                    'frac_hbm': round(cb / (rc['ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    'algorithmic_bytes': cb, 'mf_ms': round(rc['t_mf'], 5), 'kin_ms': round(rc['t_k'], 5)}
+            if not args.no_oracle:
+                import oracle as O
+                y1r, y2r, dr = oracle_chain(O, wc, rc['x'])
+                line['errors'] = {'y1': errs(rc['y1'], y1r), 'y2': errs(rc['y2'], y2r),
+                                  'ekin_rel': float(abs(rc['ekin'].sum() - dr.sum()) / abs(dr.sum()))}
+            out[wl] = line
         out['dwt3d'] = run_dwt(torch, lc, torch.float64, args.steps, args.warmup)
         out['dwt3d']['fp32'] = run_dwt(torch, lc, torch.float32, args.steps, args.warmup)
         out['hamiltonian'] = run_hamiltonian(torch, lc, torch.float64, args.steps, args.warmup)

@@ -113,6 +113,13 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
     break;
       LC_SUM2_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, PD, U, WPE, ZC)                                                       \
+  case I:                                                                                           \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d, %d>",                          \
+             batch ? "sump_batch" : "sump_fused", r, filter_tag(id), TX, TY, QX, QY, PD, U, WPE);    \
+    break;
+      LC_SUMP_VARIANTS(X)
+#undef X
     }
   } else {
     switch (v) {
@@ -136,6 +143,13 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
              r, filter_tag(id), TX, TY, QX, QY, PD, WPE);                                            \
     break;
       LC_SEPP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC)                                                              \
+  case I:                                                                                           \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>", batch ? "sepr_batch" : "sepr_fused", \
+             r, filter_tag(id), TX, TY, QX, QY, PD, WPE);                                            \
+    break;
+      LC_SEPR_VARIANTS(X)
 #undef X
     }
   }

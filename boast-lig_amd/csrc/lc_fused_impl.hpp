@@ -15,6 +15,8 @@
 #include "lc_body_sum2.hpp"
 #include "lc_body_sep2.hpp"
 #include "lc_body_sepp.hpp"
+#include "lc_body_sepr.hpp"
+#include "lc_body_sump.hpp"
 
 namespace lc {
 
@@ -205,6 +207,85 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
   return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
 }
 
+// ---- pipelined kinetic form (lc_body_sump.hpp)
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int U, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sump_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[SumpGeo<R, F, TX, TY, QX, QY, PD, U>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sump_body<R, F, TX, TY, QX, QY, PD, U>(p, bx, by, zc, arr,
+                                          (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int U, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sump_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[SumpGeo<R, F, TX, TY, QX, QY, PD, U>::lds_bytes()];
+  sump_body<R, F, TX, TY, QX, QY, PD, U>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
+}
+
+// z chunk of the kinetic forms that read planes through buffer records; ZC as
+// in launch_sum3d2 (0: pick_zchunk, < 0: at most -ZC blocks on any CU, > 0:
+// chunks down to ZC planes for ~8 blocks per CU), rounded to a multiple of Q
+template <class F, int ZC, int Q>
+int sum_zchunk(long tiles, int nz, int narr) {
+  if (ZC == 0) return pick_zchunk(tiles, nz, std::max(narr, 1), F::N, num_cu());
+  const long per = tiles * std::max(narr, 1);
+  long chunks;
+  if (ZC < 0) {
+    chunks = (long)(-ZC) * num_cu() / per;
+    if (chunks * per < num_cu()) chunks = (num_cu() + per - 1) / per;
+    chunks = std::max(1L, std::min<long>(chunks, std::max(1, nz / 16)));
+  } else {
+    chunks = (8L * num_cu() + per - 1) / per;
+    chunks = std::max(1L, std::min<long>(chunks, std::max(1, nz / ZC)));
+    if (chunks * per < num_cu())
+      chunks = std::max(chunks, std::min<long>((num_cu() + per - 1) / per, std::max(1, nz / 16)));
+  }
+  const int zc = (int)((nz + chunks - 1) / chunks);
+  return (zc + Q - 1) / Q * Q;
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int U, int WPE, int ZC>
+hipError_t launch_sump(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
+                       double* dot, hipStream_t st, double* scratch, size_t scratch_bytes) {
+  if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
+  FusedArgs<R> p{};
+  p.x = x; p.y = y;
+  p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+  p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+  p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+  p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+  p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+  p.narr = narr;
+  p.tiles_x = (p.d0.nout + TX - 1) / TX;
+  p.tiles_y = (p.d1.nout + TY - 1) / TY;
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = sum_zchunk<F, ZC, 1>(tiles, p.d2.nout, narr);
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  p.a0 = (R)a[0]; p.a1 = (R)a[1]; p.a2 = (R)a[2];
+  p.a_x = a_x; p.a_y = a_y;
+  p.flags = (a_x != R(0) ? F_AX : 0) | (a_y != R(0) ? F_AY : 0) | (dot ? F_DOT : 0);
+  p.nparts = tiles * zch;
+  if (dot) {
+    if (scratch_bytes < sizeof(double) * 3 * (size_t)p.nparts) return hipErrorInvalidValue;
+    p.part = scratch;
+  }
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  sump_fused<R, F, TX, TY, QX, QY, PD, U, WPE><<<grid, TX * TY, 0, st>>>(p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !dot) return e;
+  return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
+}
+
 // ---- two-plane separable form (lc_body_sep2.hpp)
 template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int PD>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sep3d2_fused(
@@ -337,6 +418,58 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   }
 }
 
+// ---- ring-owning separable form (lc_body_sepr.hpp), 16-tap filters only
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+__global__ __launch_bounds__((SeprGeo<R, F, TX, TY, QX, QY, PD>::NT))
+__attribute__((amdgpu_waves_per_eu(WPE))) void sepr_fused(FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[SeprGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sepr_body<R, F, TX, TY, QX, QY, PD>(p, bx, by, zc, arr, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+__global__ __launch_bounds__((SeprGeo<R, F, TX, TY, QX, QY, PD>::NT))
+__attribute__((amdgpu_waves_per_eu(WPE))) void sepr_batch(const FusedArgs<R>* __restrict__ tab,
+                                                           const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[SeprGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
+  sepr_body<R, F, TX, TY, QX, QY, PD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE, int ZC>
+hipError_t launch_sepr(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                       const R* pot, R* rho) {
+  if constexpr (!sepp_filter<F>()) {
+    return hipErrorInvalidValue;
+  } else {
+    if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
+    FusedArgs<R> p;
+    const int zch = sep_plane_args<R, F, TX, TY, ZC>(b, narr, x, y, a, a_y, pot, rho, p);
+    if (narr == 0) return hipSuccess;
+    dim3 grid((unsigned)(p.tiles_x * p.tiles_y), (unsigned)zch, (unsigned)narr);
+    sepr_fused<R, F, TX, TY, QX, QY, PD, WPE>
+        <<<grid, SeprGeo<R, F, TX, TY, QX, QY, PD>::NT, 0, st>>>(p);
+    return hipGetLastError();
+  }
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+hipError_t launch_sepr_batch(const FusedArgs<R>* tab, const BatchStage* s, hipStream_t st) {
+  if constexpr (!sepp_filter<F>()) {
+    return hipErrorInvalidValue;
+  } else {
+    sepr_batch<R, F, TX, TY, QX, QY, PD, WPE>
+        <<<s->blocks, SeprGeo<R, F, TX, TY, QX, QY, PD>::NT, 0, st>>>(tab, s->first, s->count);
+    return hipSuccess;
+  }
+}
+
 // ====================================================================== variants
 // Compiled variants of the two fused forms: tile TX x TY, CPT rows per lane,
 // QX outputs per x-task, BZ planes per window slide, HD / AD prefetch depth.
@@ -365,7 +498,8 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(22, 16, 16, 4, 4, 2, 4, 8, 16) X(23, 16, 16, 4, 4, 2, 2, 0, 16) X(24, 32, 8, 4, 4, 2, 3, 0, 16)    \
   X(25, 32, 8, 4, 4, 2, 2, 0, 16) X(26, 16, 16, 4, 4, 2, 2, 0, 64) X(27, 16, 16, 4, 4, 2, 4, 8, 64)    \
   X(28, 32, 16, 8, 8, 2, 2, 0, 0) X(29, 32, 16, 8, 8, 2, 2, 0, 32) X(30, 32, 16, 8, 8, 2, 2, 8, 32)   \
-  X(31, 32, 16, 8, 8, 2, 2, 0, -2) X(32, 32, 16, 8, 8, 4, 2, 8, -2)
+  X(31, 32, 16, 8, 8, 2, 2, 0, -2) X(32, 32, 16, 8, 8, 4, 2, 8, -2) X(33, 32, 16, 8, 8, 2, 1, 0, -1)    \
+  X(34, 32, 16, 8, 8, 4, 1, 0, -1) X(35, 16, 16, 4, 4, 2, 1, 0, -1) X(36, 32, 16, 8, 8, 2, 2, 0, -3)
 // two-plane separable form (sep only): tile, QX / QY outputs per task, QZ
 // planes per step, waves per EU, shortest z chunk (0: 4 W, ~2 blocks per CU;
 // else chunks down to ZC planes for ~4 blocks per CU), prefetch depth in steps
@@ -385,9 +519,25 @@ hipError_t launch_sepp_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(37, 32, 16, 4, 2, 2, 4, 0) X(38, 32, 16, 4, 2, 2, 6, 0) X(39, 32, 16, 4, 4, 2, 4, 0)            \
   X(40, 32, 16, 4, 2, 1, 6, 0) X(41, 32, 16, 4, 2, 4, 4, 0) X(42, 32, 16, 8, 2, 2, 4, 0)            \
   X(43, 32, 32, 4, 2, 2, 4, 0) X(44, 32, 16, 4, 2, 2, 4, 32) X(45, 32, 16, 4, 2, 2, 4, -2)          \
-  X(46, 32, 16, 4, 2, 2, 6, -2) X(47, 32, 16, 4, 2, 2, 4, -3) X(48, 32, 16, 4, 2, 2, 4, -4)
-constexpr int kFusedVariantCount = 33;
-constexpr int kSepVariantCount = 49;
+  X(46, 32, 16, 4, 2, 2, 6, -2) X(47, 32, 16, 4, 2, 2, 4, -3) X(48, 32, 16, 4, 2, 2, 4, -4)            \
+  X(60, 32, 16, 4, 2, 4, 4, -2) X(61, 32, 16, 4, 2, 8, 2, -2)
+// ring-owning separable form (sep only, 16-tap filters): tile, QX / QY
+// outputs per x- / y-task, prefetch depth in planes, waves per EU, z chunk
+// policy as above
+#define LC_SEPR_VARIANTS(X)                                                                       \
+  X(49, 32, 16, 4, 2, 2, 2, -2) X(50, 32, 16, 4, 2, 2, 4, -2) X(51, 32, 16, 4, 2, 2, 2, -3)          \
+  X(52, 32, 16, 8, 4, 2, 2, -2) X(53, 32, 16, 8, 4, 2, 4, -4) X(54, 64, 16, 4, 2, 2, 1, -1)          \
+  X(55, 32, 16, 4, 2, 2, 3, 0) X(56, 32, 16, 4, 2, 4, 2, -2) X(57, 32, 16, 4, 2, 8, 2, -2)          \
+  X(58, 32, 16, 8, 4, 4, 2, -2) X(59, 32, 16, 8, 4, 8, 2, -2)
+// pipelined kinetic form (sum only): tile, QX / QY outputs per task, halo
+// prefetch depth in planes, steps per window slide, waves per EU, z chunk
+// policy as above
+#define LC_SUMP_VARIANTS(X)                                                                           \
+  X(37, 32, 16, 2, 2, 2, 4, 4, -2) X(38, 32, 16, 4, 4, 2, 4, 4, -2) X(39, 32, 16, 2, 2, 2, 4, 2, -1)     \
+  X(40, 32, 16, 4, 4, 2, 4, 2, -1) X(41, 32, 16, 2, 2, 2, 4, 3, -2) X(42, 32, 16, 2, 2, 2, 8, 4, -2)     \
+  X(43, 32, 16, 4, 4, 2, 4, 4, -3) X(44, 32, 8, 2, 2, 2, 4, 4, -4) X(45, 32, 16, 4, 4, 2, 4, 1, -1)
+constexpr int kFusedVariantCount = 46;
+constexpr int kSepVariantCount = 62;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
@@ -404,6 +554,10 @@ hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const
 #define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC) \
   case I: return launch_sum3d2<R, F, TX, TY, QX, QY, QZ, WPE, KC, ZC>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
     LC_SUM2_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, U, WPE, ZC) \
+  case I: return launch_sump<R, F, TX, TY, QX, QY, PD, U, WPE, ZC>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    LC_SUMP_VARIANTS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -429,6 +583,10 @@ hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, 
 #define X(I, TX, TY, QX, QY, PD, WPE, ZC) \
   case I: return launch_sepp<R, F, TX, TY, QX, QY, PD, WPE, ZC>(b, narr, x, y, a, a_y, st, pot, rho);
     LC_SEPP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC) \
+  case I: return launch_sepr<R, F, TX, TY, QX, QY, PD, WPE, ZC>(b, narr, x, y, a, a_y, st, pot, rho);
+    LC_SEPR_VARIANTS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -512,6 +670,10 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
   case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
         LC_SUM2_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, PD, U, WPE, ZC) \
+  case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
+        LC_SUMP_VARIANTS(X)
+#undef X
       }
     } else {
       switch (s->variant) {
@@ -528,6 +690,7 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     return sepp_filter<F>() && items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) \
                                                     : hipErrorInvalidValue;
         LC_SEPP_VARIANTS(X)
+        LC_SEPR_VARIANTS(X)
 #undef X
       }
     }
@@ -548,6 +711,12 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     sum3d2_batch<R, F, TX, TY, QX, QY, QZ, WPE, KC><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
       LC_SUM2_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, U, WPE, ZC)                                                         \
+  case I:                                                                                            \
+    sump_batch<R, F, TX, TY, QX, QY, PD, U, WPE><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SUMP_VARIANTS(X)
 #undef X
       default: return hipErrorInvalidValue;
     }
@@ -575,6 +744,14 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     break;                                                                        \
   }
       LC_SEPP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC)                                       \
+  case I: {                                                                       \
+    const hipError_t eb = launch_sepr_batch<R, F, TX, TY, QX, QY, PD, WPE>(tab, s, st); \
+    if (eb != hipSuccess) return eb;                                              \
+    break;                                                                        \
+  }
+      LC_SEPR_VARIANTS(X)
 #undef X
       default: return hipErrorInvalidValue;
     }

@@ -77,6 +77,48 @@ def num_threads():
     return lib().lco_num_threads()
 
 
+_cpu = None
+
+
+def cpu():
+    """liblc_cpu.so: the CPU baseline (generated sweep structure, C++/OpenMP)."""
+    global _cpu
+    if _cpu is None:
+        path = os.path.join(HERE, 'liblc_cpu.so')
+        if not os.path.exists(path):
+            raise RuntimeError('CPU baseline not built: run make -C oracle')
+        L = ctypes.CDLL(path)
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.lcc_magicfilter3d.argtypes = [ip, ip, ip, ip, dp, dp, dp, dp]
+        L.lcc_kinetic3d.argtypes = [ip, ip, ip, ip, dp, dp, dp, dp]
+        L.lcc_magicfilter3d.restype = L.lcc_kinetic3d.restype = L.lcc_num_threads.restype = ctypes.c_int
+        _cpu = L
+    return _cpu
+
+
+def cpu_threads():
+    return cpu().lcc_num_threads()
+
+
+def cpu_magicfilter3d(n, bc, nx, ny, x, y, w1, w2):
+    """3D MagicFilter as three sweeps (the CPU baseline); w1 >= ny0 nx1 nx2,
+    w2 >= ny0 ny1 nx2 values, owned by the caller."""
+    (n_, pn), (b_, pb), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(bc), _iarr(nx), _iarr(ny)
+    _check(cpu().lcc_magicfilter3d(pn, pb, pnx, pny, _darr(x), _darr(y), _darr(w1), _darr(w2)),
+           'cpu_magicfilter3d')
+
+
+def cpu_kinetic3d(n, bc, nx, ny, a, x, y):
+    """3D Kinetic sum form with energies (the CPU baseline); returns the dots."""
+    (n_, pn), (b_, pb), (nx_, pnx), (ny_, pny) = _iarr(n), _iarr(bc), _iarr(nx), _iarr(ny)
+    av = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    dot = np.zeros(3)
+    _check(cpu().lcc_kinetic3d(pn, pb, pnx, pny, _darr(av), _darr(x), _darr(y), _darr(dot)),
+           'cpu_kinetic3d')
+    return dot
+
+
 def taps(fil):
     out = np.zeros(64)
     lo = ctypes.c_int()
