@@ -1214,6 +1214,10 @@ int tune_stages(lc_batch* b, int repeat, double* ms_out) {
     for (int v = 0; v < variant_count(form) && rc == LC_OK; ++v) {
       if (!has_batch_form(form, v)) continue;
       hipError_t er = rebuild_stage<R>(b, si, first, v);
+      if (er == hipErrorInvalidValue) {  // the variant does not take these items (the stage is unchanged)
+        (void)hipGetLastError();
+        continue;
+      }
       if (er != hipSuccess) { rc = hip_fail(er, "lc_batch_tune"); break; }
       double tmin = 1e300;
       for (int r = 0; r <= std::max(1, repeat); ++r) {  // first run untimed

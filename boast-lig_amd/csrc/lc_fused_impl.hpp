@@ -16,6 +16,7 @@
 #include "lc_body_sep2.hpp"
 #include "lc_body_sepp.hpp"
 #include "lc_body_sepr.hpp"
+#include "lc_body_sepq.hpp"
 #include "lc_body_sump.hpp"
 
 namespace lc {
@@ -470,6 +471,61 @@ hipError_t launch_sepr_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   }
 }
 
+// ---- 128-bit-LDS separable form (lc_body_sepq.hpp), 16-tap filters only
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+__global__ __launch_bounds__((SepqGeo<R, F, TX, TY, QX, QY, PD>::NT))
+__attribute__((amdgpu_waves_per_eu(WPE))) void sepq_fused(FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[SepqGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sepq_body<R, F, TX, TY, QX, QY, PD>(p, bx, by, zc, arr, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+__global__ __launch_bounds__((SepqGeo<R, F, TX, TY, QX, QY, PD>::NT))
+__attribute__((amdgpu_waves_per_eu(WPE))) void sepq_batch(const FusedArgs<R>* __restrict__ tab,
+                                                           const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[SepqGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
+  sepq_body<R, F, TX, TY, QX, QY, PD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY>
+constexpr bool sepq_ok() { return sepp_filter<F>() && sepq_shape_ok<R, TX, TY, QX, QY>(); }
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE, int ZC>
+hipError_t launch_sepq(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st,
+                       const R* pot, R* rho) {
+  if constexpr (!sepq_ok<R, F, TX, TY, QX, QY>()) {
+    return hipErrorInvalidValue;
+  } else {
+    if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
+    FusedArgs<R> p;
+    const int zch = sep_plane_args<R, F, TX, TY, ZC>(b, narr, x, y, a, a_y, pot, rho, p);
+    if (narr == 0) return hipSuccess;
+    dim3 grid((unsigned)(p.tiles_x * p.tiles_y), (unsigned)zch, (unsigned)narr);
+    sepq_fused<R, F, TX, TY, QX, QY, PD, WPE>
+        <<<grid, SepqGeo<R, F, TX, TY, QX, QY, PD>::NT, 0, st>>>(p);
+    return hipGetLastError();
+  }
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
+hipError_t launch_sepq_batch(const FusedArgs<R>* tab, const BatchStage* s, hipStream_t st) {
+  if constexpr (!sepq_ok<R, F, TX, TY, QX, QY>()) {
+    return hipErrorInvalidValue;
+  } else {
+    sepq_batch<R, F, TX, TY, QX, QY, PD, WPE>
+        <<<s->blocks, SepqGeo<R, F, TX, TY, QX, QY, PD>::NT, 0, st>>>(tab, s->first, s->count);
+    return hipSuccess;
+  }
+}
+
 // ====================================================================== variants
 // Compiled variants of the two fused forms: tile TX x TY, CPT rows per lane,
 // QX outputs per x-task, BZ planes per window slide, HD / AD prefetch depth.
@@ -536,8 +592,14 @@ hipError_t launch_sepr_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
   X(37, 32, 16, 2, 2, 2, 4, 4, -2) X(38, 32, 16, 4, 4, 2, 4, 4, -2) X(39, 32, 16, 2, 2, 2, 4, 2, -1)     \
   X(40, 32, 16, 4, 4, 2, 4, 2, -1) X(41, 32, 16, 2, 2, 2, 4, 3, -2) X(42, 32, 16, 2, 2, 2, 8, 4, -2)     \
   X(43, 32, 16, 4, 4, 2, 4, 4, -3) X(44, 32, 8, 2, 2, 2, 4, 4, -4) X(45, 32, 16, 4, 4, 2, 4, 1, -1)
+// 128-bit-LDS separable form (sep only, 16-tap filters; QX and QY whole
+// 16-byte pieces, so some entries take one precision only): tile, QX / QY,
+// prefetch depth, waves per EU, z chunk policy
+#define LC_SEPQ_VARIANTS(X)                                                                       \
+  X(62, 32, 16, 4, 2, 2, 2, -2) X(63, 32, 16, 4, 2, 2, 4, -2) X(64, 32, 16, 8, 4, 2, 2, -2)          \
+  X(65, 32, 16, 8, 4, 2, 4, -4) X(66, 64, 16, 8, 4, 2, 2, -2) X(67, 32, 16, 4, 2, 2, 2, -3)
 constexpr int kFusedVariantCount = 46;
-constexpr int kSepVariantCount = 62;
+constexpr int kSepVariantCount = 68;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
@@ -587,6 +649,10 @@ hipError_t sep3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, 
 #define X(I, TX, TY, QX, QY, PD, WPE, ZC) \
   case I: return launch_sepr<R, F, TX, TY, QX, QY, PD, WPE, ZC>(b, narr, x, y, a, a_y, st, pot, rho);
     LC_SEPR_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC) \
+  case I: return launch_sepq<R, F, TX, TY, QX, QY, PD, WPE, ZC>(b, narr, x, y, a, a_y, st, pot, rho);
+    LC_SEPQ_VARIANTS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -692,6 +758,12 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
         LC_SEPP_VARIANTS(X)
         LC_SEPR_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC)                                                                  \
+  case I:                                                                                                 \
+    return sepq_ok<R, F, TX, TY, QX, QY>() && items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) \
+                                                                 : hipErrorInvalidValue;
+        LC_SEPQ_VARIANTS(X)
+#undef X
       }
     }
     return hipErrorInvalidValue;
@@ -752,6 +824,14 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     break;                                                                        \
   }
       LC_SEPR_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, PD, WPE, ZC)                                       \
+  case I: {                                                                       \
+    const hipError_t eb = launch_sepq_batch<R, F, TX, TY, QX, QY, PD, WPE>(tab, s, st); \
+    if (eb != hipSuccess) return eb;                                              \
+    break;                                                                        \
+  }
+      LC_SEPQ_VARIANTS(X)
 #undef X
       default: return hipErrorInvalidValue;
     }

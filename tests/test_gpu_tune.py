@@ -25,11 +25,11 @@ def lc(gpu):
     libconv.tune_clear()
 
 
-@pytest.mark.parametrize('v', range(62))
+@pytest.mark.parametrize('v', range(68))
 @pytest.mark.parametrize('bc', [(1, -1, 0), (2, 0, 2)])
 def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v, bc):
     torch = gpu
-    assert max(lc.variant_count(lc.FORM_SEP), lc.variant_count(lc.FORM_SUM)) == 62
+    assert max(lc.variant_count(lc.FORM_SEP), lc.variant_count(lc.FORM_SUM)) == 68
     rng = np.random.default_rng(1200 + v)
     n = [37, 21, 45]
     bc = list(bc)
@@ -93,10 +93,18 @@ def test_all_separable_variants_give_identical_bits(lc, gpu, monkeypatch):
     for dt in (torch.float64, torch.float32):
         x = torch.from_numpy(rng.uniform(-1, 1, int(np.prod(nx)))).to('cuda', dt)
         outs = []
+        prec = 8 if dt == torch.float64 else 4
         for v in range(lc.variant_count(lc.FORM_SEP)):
             monkeypatch.setenv('LIBCONV_SEP3D_VARIANT', str(v))
             y = torch.zeros(int(np.prod(ny)), dtype=dt, device='cuda')
-            lc.conv_md(lc.FIL_MD, n, bc, nx, ny, 1, x, y, a=0.5)
+            try:
+                lc.conv_md(lc.FIL_MD, n, bc, nx, ny, 1, x, y, a=0.5)
+            except lc.LibconvError:
+                # the 128-bit-LDS kernels take whole 16-byte pieces per task:
+                # some of their shapes exist in one precision only
+                name = lc.kernel_name(prec, lc.FORM_SEP, lc.FIL_MD, n, bc, nx, ny)
+                assert name.startswith('sepq_fused'), (v, name)
+                continue
             outs.append(y)
         torch.cuda.synchronize()
         for v, y in enumerate(outs):

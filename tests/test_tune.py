@@ -17,7 +17,7 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 
 
 def test_variant_counts(lc):
-    assert lc.variant_count(lc.FORM_SEP) == 62
+    assert lc.variant_count(lc.FORM_SEP) == 68
     assert lc.variant_count(lc.FORM_SUM) == 46
     assert lc.variant_count(lc.FORM_DWT) == 7
     assert lc.variant_count(lc.FORM_IDWT) == 7
@@ -27,9 +27,9 @@ def test_default_kernel_names(lc):
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<double, lc::FD2, 32, 16, 8, 8, 2, 2, 0>'
     assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<float, lc::FD2, 32, 16, 8, 8, 4, 2, 8>'
     assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
-        'sepp_fused<float, lc::FMD, 32, 16, 4, 2, 2, 4>'
+        'sepq_fused<float, lc::FMD, 64, 16, 8, 4, 2, 2>'
     assert lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
-        'sepp_fused<double, lc::FMD, 32, 16, 4, 2, 2, 4>'
+        'sepq_fused<double, lc::FMD, 32, 16, 4, 2, 2, 2>'
     assert lc.kernel_name(8, lc.FORM_IDWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).startswith(
         'sepw3d_fused<double, lc::Idwt<')
 
@@ -41,7 +41,7 @@ def test_planes_beyond_buffer_records_use_pointer_kernels(lc):
     big = dict(n=[16400, 16400, 8], bc=0, nx=[16400, 16400, 8], ny=[16400, 16400, 8])
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **big).startswith('sum3d_fused<double')
     assert lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, **big).startswith('sep3d_fused<double')
-    assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, **big).startswith('sepp_fused<float')
+    assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, **big).startswith('sepq_fused<float')
 
 
 def test_environment_selects_variant(lc, monkeypatch):
