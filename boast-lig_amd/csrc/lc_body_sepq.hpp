@@ -44,6 +44,7 @@ __device__ __forceinline__ void lds_read16(const R* p, R* v) {
 
 constexpr int odd_ceil(int v) { return v % 2 ? v : v + 1; }
 constexpr int imax(int a, int b) { return a > b ? a : b; }
+constexpr int igcd(int a, int b) { return b ? igcd(b, a % b) : a; }
 
 // the shapes this kernel takes: whole 16-byte pieces per task start
 template <typename R, int TX, int TY, int QX, int QY>
@@ -55,7 +56,10 @@ template <typename R, class F, int TX, int TY, int QX, int QY, int PD>
 struct SepqGeo {
   static constexpr int W = F::N, V = Vec16<R>::N;
   static constexpr int TWD = TX + W - 1, TH = TY + W - 1;
-  static constexpr int AR = (TH + 31) / 32 * 32;          // rows of A (x-tasks per group)
+  // rows of A = x-tasks per column group: the fewest rows >= TH that make
+  // the x-tasks whole waves (TX / QX column groups of AR rows each)
+  static constexpr int ARS = 64 / igcd(64, TX / QX);
+  static constexpr int AR = (TH + ARS - 1) / ARS * ARS;
   static constexpr int RX = (QX + W - 1 + V - 1) / V;     // 16-byte reads per x-task
   static constexpr int RY = (QY + W - 1 + V - 1) / V;     // 16-byte reads per y-task
   // row stride of A and column stride of B: odd multiples of 16 bytes that

@@ -45,8 +45,12 @@ struct Sum2Geo {
 };
 
 // KC > 0: a scheduling fence every KC tile reads of a task bounds the values
-// in flight (fewer VGPRs, more waves per SIMD)
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int KC>
+// in flight (fewer VGPRs, more waves per SIMD).  PW: steps of column planes in
+// flight; the column loads are the ones that miss L2 (each plane's column
+// value is first needed 14 planes ahead of its tile), so PW > 1 issues them
+// PW steps ahead, with the step loop unrolled PW times so that every
+// in-flight register has a static slot.
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int KC, int PW = 1>
 __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
                                             long bid, unsigned char* lds) {
   using G = Sum2Geo<R, F, TX, TY, QX, QY, QZ>;
@@ -125,106 +129,114 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
   R win[WS];
 #pragma unroll
   for (int s = 0; s < W - 1; ++s) win[s] = col_next();  // planes zb + LO .. zb + UP - 1
-  R nxt[QZ];
+  R nxt[PW][QZ];  // the window's next planes, PW steps of them
 #pragma unroll
-  for (int q = 0; q < QZ; ++q) nxt[q] = col_next();
+  for (int w = 0; w < PW; ++w)
+#pragma unroll
+    for (int q = 0; q < QZ; ++q) nxt[w][q] = col_next();
   R hreg[NHT];
   load_halo(hreg);
 
   const bool want_dot = (p.flags & F_DOT) && arr == p.narr - 1;
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
 
-  for (int z0 = zb; z0 < ze; z0 += QZ) {
-    // window gains the planes z0 + UP .. z0 + UP + QZ - 1
+  for (int zu = zb; zu < ze; zu += PW * QZ) {
 #pragma unroll
-    for (int q = 0; q < QZ; ++q) win[W - 1 + q] = nxt[q];
-    // tile of planes z0 .. z0+QZ-1: centre from the window, halo from registers
+    for (int u = 0; u < PW; ++u) {
+      const int z0 = zu + u * QZ;
+      if (PW > 1 && z0 >= ze) break;  // uniform
+      // window gains the planes z0 + UP .. z0 + UP + QZ - 1
 #pragma unroll
-    for (int q = 0; q < QZ; ++q) T[(q * TH + ty + HL) * TW + tx + HL] = win[HL + q];
+      for (int q = 0; q < QZ; ++q) win[W - 1 + q] = nxt[u][q];
+      // tile of planes z0 .. z0+QZ-1: centre from the window, halo from registers
 #pragma unroll
-    for (int q = 0; q < QZ; ++q)
+      for (int q = 0; q < QZ; ++q) T[(q * TH + ty + HL) * TW + tx + HL] = win[HL + q];
 #pragma unroll
-      for (int j = 0; j < NHL; ++j)
-        T[q * TH * TW + hslot[j]] = hreg[q * NHL + j];
-    // prefetch the next step (after the last step: planes nobody uses)
+      for (int q = 0; q < QZ; ++q)
 #pragma unroll
-    for (int q = 0; q < QZ; ++q) nxt[q] = col_next();
-    load_halo(hreg);
-    __syncthreads();
-    // ---- one task per lane: x-outputs of a row or y-outputs of a column
-    if (t < G::NXT) {
-      // consecutive lanes on consecutive rows (odd stride: no bank conflicts)
-      const int r = t % (QZ * TY), c0 = (t / (QZ * TY)) * QX;
-      const int q = r / TY, rr = r % TY;
-      const R* row = T + (q * TH + rr + HL) * TW + c0;
-      R acc[QX];
+        for (int j = 0; j < NHL; ++j)
+          T[q * TH * TW + hslot[j]] = hreg[q * NHL + j];
+      // prefetch: column planes PW steps ahead, halo planes one step ahead
+      // (after the last steps: planes nobody uses)
 #pragma unroll
-      for (int i = 0; i < QX; ++i) acc[i] = R(0);
+      for (int q = 0; q < QZ; ++q) nxt[u][q] = col_next();
+      load_halo(hreg);
+      __syncthreads();
+      // ---- one task per lane: x-outputs of a row or y-outputs of a column
+      if (t < G::NXT) {
+        // consecutive lanes on consecutive rows (odd stride: no bank conflicts)
+        const int r = t % (QZ * TY), c0 = (t / (QZ * TY)) * QX;
+        const int q = r / TY, rr = r % TY;
+        const R* row = T + (q * TH + rr + HL) * TW + c0;
+        R acc[QX];
 #pragma unroll
-      for (int k = 0; k < QX + W - 1; ++k) {
-        if (KC > 0 && k > 0 && k % KC == 0) __builtin_amdgcn_sched_barrier(0);
-        const R v = row[k];
+        for (int i = 0; i < QX; ++i) acc[i] = R(0);
 #pragma unroll
-        for (int i = 0; i < QX; ++i)
-          if (k - i >= 0 && k - i < W) acc[i] = fmar(v, tap<R, F>(k - i), acc[i]);
-      }
+        for (int k = 0; k < QX + W - 1; ++k) {
+          if (KC > 0 && k > 0 && k % KC == 0) __builtin_amdgcn_sched_barrier(0);
+          const R v = row[k];
 #pragma unroll
-      for (int i = 0; i < QX; ++i) SX[(q * TY + rr) * SW + c0 + i] = acc[i];
-    } else if (t < G::NXT + G::NYT) {
-      const int u = t - G::NXT;
-      const int c = u % TX, rest = u / TX;
-      const int q = rest % QZ, r0 = (rest / QZ) * QY;
-      const R* colp = T + (q * TH + r0) * TW + c + HL;
-      R acc[QY];
-#pragma unroll
-      for (int i = 0; i < QY; ++i) acc[i] = R(0);
-#pragma unroll
-      for (int k = 0; k < QY + W - 1; ++k) {
-        if (KC > 0 && k > 0 && k % KC == 0) __builtin_amdgcn_sched_barrier(0);
-        const R v = colp[k * TW];
-#pragma unroll
-        for (int i = 0; i < QY; ++i)
-          if (k - i >= 0 && k - i < W) acc[i] = fmar(v, tap<R, F>(k - i), acc[i]);
-      }
-#pragma unroll
-      for (int i = 0; i < QY; ++i) SY[(q * TY + r0 + i) * SW + c] = acc[i];
-    }
-    __syncthreads();
-    // ---- combine, epilogue, store
-#pragma unroll
-    for (int q = 0; q < QZ; ++q) {
-      const int z = z0 + q;
-      if (z < ze) {
-        R zt = R(0);
-#pragma unroll
-        for (int k = 0; k < W; ++k) zt = fmar(win[q + k], tap<R, F>(k), zt);
-        const R xc = win[HL + q];
-        const bool has_x = out_ok && coff >= 0 && (zper || (unsigned)(z - zoff) < (unsigned)zlim);
-        R t0 = SX[(q * TY + ty) * SW + tx] * p.a0;
-        const R t1 = SY[(q * TY + ty) * SW + tx] * p.a1;
-        const R t2 = zt * p.a2;
-        if (want_dot && has_x) {
-          d0 = fmar((double)xc, (double)t0, d0);
-          d1 = fmar((double)xc, (double)t1, d1);
-          d2 = fmar((double)xc, (double)t2, d2);
+          for (int i = 0; i < QX; ++i)
+            if (k - i >= 0 && k - i < W) acc[i] = fmar(v, tap<R, F>(k - i), acc[i]);
         }
-        if ((p.flags & F_AX) && has_x) t0 = fmar(xc, p.a_x, t0);
-        R tot = t1 + t0;
-        tot = t2 + tot;
-        if (out_ok) {
-          R* yp = ya + (ocol + zo);
-          if (p.flags & F_AY) tot = fmar(p.a_y, *yp, tot);
-          // streaming store: the output is not read again by this kernel, and
-          // keeping it out of L2 leaves room for the halo values that
-          // neighbouring tiles re-read (5-8% faster at 256^3)
-          __builtin_nontemporal_store(tot, yp);
-        }
-        zo += syo;
-      }
-    }
-    // slide the window by QZ steps
 #pragma unroll
-    for (int s = 0; s < W - 1; ++s) win[s] = win[s + QZ];
+        for (int i = 0; i < QX; ++i) SX[(q * TY + rr) * SW + c0 + i] = acc[i];
+      } else if (t < G::NXT + G::NYT) {
+        const int u = t - G::NXT;
+        const int c = u % TX, rest = u / TX;
+        const int q = rest % QZ, r0 = (rest / QZ) * QY;
+        const R* colp = T + (q * TH + r0) * TW + c + HL;
+        R acc[QY];
+#pragma unroll
+        for (int i = 0; i < QY; ++i) acc[i] = R(0);
+#pragma unroll
+        for (int k = 0; k < QY + W - 1; ++k) {
+          if (KC > 0 && k > 0 && k % KC == 0) __builtin_amdgcn_sched_barrier(0);
+          const R v = colp[k * TW];
+#pragma unroll
+          for (int i = 0; i < QY; ++i)
+            if (k - i >= 0 && k - i < W) acc[i] = fmar(v, tap<R, F>(k - i), acc[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < QY; ++i) SY[(q * TY + r0 + i) * SW + c] = acc[i];
+      }
+      __syncthreads();
+      // ---- combine, epilogue, store
+#pragma unroll
+      for (int q = 0; q < QZ; ++q) {
+        const int z = z0 + q;
+        if (z < ze) {
+          R zt = R(0);
+#pragma unroll
+          for (int k = 0; k < W; ++k) zt = fmar(win[q + k], tap<R, F>(k), zt);
+          const R xc = win[HL + q];
+          const bool has_x = out_ok && coff >= 0 && (zper || (unsigned)(z - zoff) < (unsigned)zlim);
+          R t0 = SX[(q * TY + ty) * SW + tx] * p.a0;
+          const R t1 = SY[(q * TY + ty) * SW + tx] * p.a1;
+          const R t2 = zt * p.a2;
+          if (want_dot && has_x) {
+            d0 = fmar((double)xc, (double)t0, d0);
+            d1 = fmar((double)xc, (double)t1, d1);
+            d2 = fmar((double)xc, (double)t2, d2);
+          }
+          if ((p.flags & F_AX) && has_x) t0 = fmar(xc, p.a_x, t0);
+          R tot = t1 + t0;
+          tot = t2 + tot;
+          if (out_ok) {
+            R* yp = ya + (ocol + zo);
+            if (p.flags & F_AY) tot = fmar(p.a_y, *yp, tot);
+            // streaming store: the output is not read again by this kernel, and
+            // keeping it out of L2 leaves room for the halo values that
+            // neighbouring tiles re-read (5-8% faster at 256^3)
+            __builtin_nontemporal_store(tot, yp);
+          }
+          zo += syo;
+        }
+      }
+      // slide the window by QZ steps
+#pragma unroll
+      for (int s = 0; s < W - 1; ++s) win[s] = win[s + QZ];
+    }
   }
   if (p.part) {
     double s = wave_block_sum<R>(d0, red, NT);

@@ -120,6 +120,13 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
     break;
       LC_SUMP_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC, PW)                                                   \
+  case I:                                                                                           \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d, %d, %d>",                      \
+             batch ? "sum3d2w_batch" : "sum3d2w_fused", r, filter_tag(id), TX, TY, QX, QY, QZ, WPE, KC, PW); \
+    break;
+      LC_SUM2W_VARIANTS(X)
+#undef X
     }
   } else {
     switch (v) {
@@ -184,20 +191,21 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     if (v >= 0) return v;
     // defaults: the fastest variants on MI355X (MF on output planes of
     // 200^2 points and more: the 128-bit-LDS kernel, 32 x 16 tiles in FP64,
-    // 64 x 16 in FP32; on smaller planes, e.g. the 157^3 magic filter of the
+    // 32 x 32 in FP32; on smaller planes, e.g. the 157^3 magic filter of the
     // local Hamiltonian at 64^3 pairs, whose few tiles the z chunking splits
     // into short chunks: the pipelined kernel; K: the two-plane kernel with
-    // at most two blocks per CU); the plane-buffer forms (16 and up) take
-    // planes of at most kPlaneMax bytes
+    // at most two blocks per CU, in FP64 with the column planes two steps in
+    // flight); the plane-buffer forms (16 and up) take planes of at most
+    // kPlaneMax bytes
     const bool fit = plane_fits(b, prec);
     if (form == FORM_SEP) {
       if (!fit) return 12;
       const int g = FMD::N - 1;  // outputs per dim: grow adds the filter length
       const long plane = (long)(b.bc[0] == GROW ? b.n[0] + g : b.n[0]) * (b.bc[1] == GROW ? b.n[1] + g : b.n[1]);
-      return plane >= 200L * 200 ? (prec == 4 ? 66 : 62) : 45;
+      return plane >= 200L * 200 ? (prec == 4 ? 68 : 62) : 45;
     }
     if (!fit) return prec == 4 ? 7 : 8;
-    return prec == 4 ? 32 : 31;
+    return prec == 4 ? 32 : 46;
   }
   v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
   return v >= 0 ? v : 5;  // one z-step per slide, at most two blocks per CU

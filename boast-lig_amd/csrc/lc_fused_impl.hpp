@@ -154,7 +154,31 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
 }
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC, int ZC>
+// the same with the column planes PW > 1 steps in flight
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC, int PW>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sum3d2w_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW>(p, bx, by, zc, arr,
+                                            (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC, int PW>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sum3d2w_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC, int ZC, int PW = 1>
 hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
                          double* dot, hipStream_t st, double* scratch, size_t scratch_bytes) {
   if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
@@ -202,7 +226,10 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
   }
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
-  sum3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE, KC><<<grid, TX * TY, 0, st>>>(p);
+  if constexpr (PW == 1)
+    sum3d2_fused<R, F, TX, TY, QX, QY, QZ, WPE, KC><<<grid, TX * TY, 0, st>>>(p);
+  else
+    sum3d2w_fused<R, F, TX, TY, QX, QY, QZ, WPE, KC, PW><<<grid, TX * TY, 0, st>>>(p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !dot) return e;
   return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
@@ -597,9 +624,15 @@ hipError_t launch_sepq_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
 // prefetch depth, waves per EU, z chunk policy
 #define LC_SEPQ_VARIANTS(X)                                                                       \
   X(62, 32, 16, 4, 2, 2, 2, -2) X(63, 32, 16, 4, 2, 2, 4, -2) X(64, 32, 16, 8, 4, 2, 2, -2)          \
-  X(65, 32, 16, 8, 4, 2, 4, -4) X(66, 64, 16, 8, 4, 2, 2, -2) X(67, 32, 16, 4, 2, 2, 2, -3)
-constexpr int kFusedVariantCount = 46;
-constexpr int kSepVariantCount = 68;
+  X(65, 32, 16, 8, 4, 2, 4, -4) X(66, 64, 16, 8, 4, 2, 2, -2) X(67, 32, 16, 4, 2, 2, 2, -3)          \
+  X(68, 32, 32, 8, 4, 2, 2, -2) X(69, 32, 32, 4, 2, 2, 1, -1) X(70, 32, 24, 4, 2, 2, 1, -1)          \
+  X(71, 64, 32, 8, 4, 2, 1, -1) X(72, 32, 32, 8, 4, 2, 2, -3)
+// two-plane kinetic form with the column planes PW steps in flight: as
+// LC_SUM2_VARIANTS, then PW
+#define LC_SUM2W_VARIANTS(X) \
+  X(46, 32, 16, 8, 8, 2, 4, 0, -2, 2) X(47, 32, 16, 8, 8, 2, 2, 0, -2, 2) X(48, 32, 16, 8, 8, 4, 2, 8, -2, 2)
+constexpr int kFusedVariantCount = 49;
+constexpr int kSepVariantCount = 73;
 
 template <typename R, class F>
 hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const double* a,
@@ -620,6 +653,10 @@ hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const
 #define X(I, TX, TY, QX, QY, PD, U, WPE, ZC) \
   case I: return launch_sump<R, F, TX, TY, QX, QY, PD, U, WPE, ZC>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
     LC_SUMP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC, PW) \
+  case I: return launch_sum3d2<R, F, TX, TY, QX, QY, QZ, WPE, KC, ZC, PW>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    LC_SUM2W_VARIANTS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -740,6 +777,10 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
   case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
         LC_SUMP_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC, PW) \
+  case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
+        LC_SUM2W_VARIANTS(X)
+#undef X
       }
     } else {
       switch (s->variant) {
@@ -789,6 +830,12 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     sump_batch<R, F, TX, TY, QX, QY, PD, U, WPE><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
       LC_SUMP_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC, PW)                                                     \
+  case I:                                                                                            \
+    sum3d2w_batch<R, F, TX, TY, QX, QY, QZ, WPE, KC, PW><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
+    break;
+      LC_SUM2W_VARIANTS(X)
 #undef X
       default: return hipErrorInvalidValue;
     }

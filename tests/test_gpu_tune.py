@@ -25,11 +25,11 @@ def lc(gpu):
     libconv.tune_clear()
 
 
-@pytest.mark.parametrize('v', range(68))
+@pytest.mark.parametrize('v', range(73))
 @pytest.mark.parametrize('bc', [(1, -1, 0), (2, 0, 2)])
 def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v, bc):
     torch = gpu
-    assert max(lc.variant_count(lc.FORM_SEP), lc.variant_count(lc.FORM_SUM)) == 68
+    assert max(lc.variant_count(lc.FORM_SEP), lc.variant_count(lc.FORM_SUM)) == 73
     rng = np.random.default_rng(1200 + v)
     n = [37, 21, 45]
     bc = list(bc)

@@ -17,17 +17,17 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 
 
 def test_variant_counts(lc):
-    assert lc.variant_count(lc.FORM_SEP) == 68
-    assert lc.variant_count(lc.FORM_SUM) == 46
+    assert lc.variant_count(lc.FORM_SEP) == 73
+    assert lc.variant_count(lc.FORM_SUM) == 49
     assert lc.variant_count(lc.FORM_DWT) == 7
     assert lc.variant_count(lc.FORM_IDWT) == 7
 
 
 def test_default_kernel_names(lc):
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<double, lc::FD2, 32, 16, 8, 8, 2, 2, 0>'
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2w_fused<double, lc::FD2, 32, 16, 8, 8, 2, 4, 0, 2>'
     assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<float, lc::FD2, 32, 16, 8, 8, 4, 2, 8>'
     assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
-        'sepq_fused<float, lc::FMD, 64, 16, 8, 4, 2, 2>'
+        'sepq_fused<float, lc::FMD, 32, 32, 8, 4, 2, 2>'
     assert lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
         'sepq_fused<double, lc::FMD, 32, 16, 4, 2, 2, 2>'
     assert lc.kernel_name(8, lc.FORM_IDWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).startswith(
@@ -48,7 +48,7 @@ def test_environment_selects_variant(lc, monkeypatch):
     monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', '6')
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d_ws<double, lc::FD2, 64, 8, 4, 4, 2>'
     monkeypatch.setenv('LIBCONV_SUM3D_VARIANT', '99')   # out of range: default
-    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('32, 16, 8, 8, 2, 2, 0>')
+    assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256).endswith('32, 16, 8, 8, 2, 4, 0, 2>')
 
 
 def test_json_cache_round_trip(lc, tmp_path, monkeypatch):
