@@ -81,7 +81,13 @@ struct SepqGeo {
   static_assert(W % 2 == 0 && W % PD == 0, "static ring and prefetch slots");
 };
 
-template <typename R, class F, int TX, int TY, int QX, int QY, int PD>
+// ME: the epilogue reads or writes memory besides the output (F_AY, F_POT).
+// Without (ME false) every wave issues the same memory operations in every
+// step: the y-waves' outputs are stored after the x/y branch, by all waves,
+// as buffer stores that drop the x-waves' and the invalid lanes' values (out
+// of range).  The compiler's wait counts are then exact, and a step waits for
+// the tile loads it uses, not for the previous step's stores.
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, bool ME = true>
 __device__ __forceinline__ void sepq_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
                                           unsigned char* lds) {
   using G = SepqGeo<R, F, TX, TY, QX, QY, PD>;
@@ -166,6 +172,9 @@ __device__ __forceinline__ void sepq_body(const FusedArgs<R>& p, int bx, int by,
         for (int j = 0; j < NAP; ++j) bq[j] = plane_ld<R>(rs, aoff[j]);
         zc_ld.next();
       }
+      R outv[QY];  // ME false: the step's outputs of this lane
+#pragma unroll
+      for (int i = 0; i < QY; ++i) outv[i] = R(0);
       if (xwave) {
         // ---- x-task on plane s: 16-byte reads along the row
         const R* row = A + (u & 1) * HA + xo;
@@ -208,23 +217,44 @@ __device__ __forceinline__ void sepq_body(const FusedArgs<R>& p, int bx, int by,
 #pragma unroll
         for (int i = 0; i < QY; ++i) ring[i][(u + W - 1) % W] = acc[i];
         // ---- z: the outputs whose planes s - W .. s - 1 are complete
-        if (s >= W) {
+        if constexpr (ME) {
+          if (s >= W) {
+#pragma unroll
+            for (int i = 0; i < QY; ++i) {
+              R sum = R(0);
+#pragma unroll
+              for (int k = 0; k < W; ++k) sum = fmar(ring[i][(u + k) % W], tap<R, F>(k), sum);
+              if (p.flags & F_A) sum = sum * p.a0;
+              if (xin && i < yrows) {
+                const long o = ocol + (long)i * p.d0.ny + zo;
+                R* yp = ya + o;
+                if (p.flags & F_AY) sum = fmar(p.a_y, *yp, sum);
+                if (p.flags & F_POT) sum = pot_step(p, o, sum);
+                *yp = sum;
+              }
+            }
+            zo += syo;
+          }
+        } else {
 #pragma unroll
           for (int i = 0; i < QY; ++i) {
             R sum = R(0);
 #pragma unroll
             for (int k = 0; k < W; ++k) sum = fmar(ring[i][(u + k) % W], tap<R, F>(k), sum);
             if (p.flags & F_A) sum = sum * p.a0;
-            if (xin && i < yrows) {
-              const long o = ocol + (long)i * p.d0.ny + zo;
-              R* yp = ya + o;
-              if (p.flags & F_AY) sum = fmar(p.a_y, *yp, sum);
-              if (p.flags & F_POT) sum = pot_step(p, o, sum);
-              *yp = sum;
-            }
+            outv[i] = sum;
           }
-          zo += syo;
         }
+      }
+      if constexpr (!ME) {
+        // the outputs of planes s - W (before plane W: none), stored by every wave
+        const bool em = s >= W;  // uniform
+        const __amdgpu_buffer_rsrc_t ro = span_rsrc(ya + zo, em ? syo : 0);
+#pragma unroll
+        for (int i = 0; i < QY; ++i)
+          plane_st<R>(ro, (!xwave && xin && i < yrows) ? (int)((ocol + (long)i * p.d0.ny) * (long)sizeof(R)) : kOffNone,
+                      outv[i]);
+        zo += em ? syo : 0;
       }
       __syncthreads();
       if (s + 1 == nsteps) return;  // the last output plane (q = np - 1) is stored

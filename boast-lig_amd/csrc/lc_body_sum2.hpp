@@ -49,8 +49,13 @@ struct Sum2Geo {
 // flight; the column loads are the ones that miss L2 (each plane's column
 // value is first needed 14 planes ahead of its tile), so PW > 1 issues them
 // PW steps ahead, with the step loop unrolled PW times so that every
-// in-flight register has a static slot.
-template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int KC, int PW = 1>
+// in-flight register has a static slot.  ME: the epilogue reads y (F_AY).
+// Without (ME false) every step issues the same memory operations: the
+// outputs go out as buffer stores that drop the lanes outside the output
+// and the planes past the chunk (out of range), and the set-up ends with as
+// many dropped stores as a step, so the compiler's wait counts are exact and
+// a step waits for the loads it uses, not for the previous step's stores.
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int KC, int PW = 1, bool ME = true>
 __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
                                             long bid, unsigned char* lds) {
   using G = Sum2Geo<R, F, TX, TY, QX, QY, QZ>;
@@ -129,13 +134,22 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
   R win[WS];
 #pragma unroll
   for (int s = 0; s < W - 1; ++s) win[s] = col_next();  // planes zb + LO .. zb + UP - 1
-  R nxt[PW][QZ];  // the window's next planes, PW steps of them
+  // the window's next planes, PW steps of them, and the first step's halo,
+  // issued in the order of a step's loads (halo, then the far column planes)
+  R nxt[PW][QZ];
+  R hreg[NHT];
 #pragma unroll
-  for (int w = 0; w < PW; ++w)
+  for (int w = 0; w < PW - 1; ++w)
 #pragma unroll
     for (int q = 0; q < QZ; ++q) nxt[w][q] = col_next();
-  R hreg[NHT];
   load_halo(hreg);
+#pragma unroll
+  for (int q = 0; q < QZ; ++q) nxt[PW - 1][q] = col_next();
+  if constexpr (!ME) {
+    const __amdgpu_buffer_rsrc_t none = span_rsrc(ya, 0);
+#pragma unroll
+    for (int q = 0; q < QZ; ++q) plane_st<R, true>(none, kOffNone + q * (int)sizeof(R), R(0));
+  }
 
   const bool want_dot = (p.flags & F_DOT) && arr == p.narr - 1;
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
@@ -156,11 +170,11 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
 #pragma unroll
         for (int j = 0; j < NHL; ++j)
           T[q * TH * TW + hslot[j]] = hreg[q * NHL + j];
-      // prefetch: column planes PW steps ahead, halo planes one step ahead
+      // prefetch: halo planes one step ahead, column planes PW steps ahead
       // (after the last steps: planes nobody uses)
+      load_halo(hreg);
 #pragma unroll
       for (int q = 0; q < QZ; ++q) nxt[u][q] = col_next();
-      load_halo(hreg);
       __syncthreads();
       // ---- one task per lane: x-outputs of a row or y-outputs of a column
       if (t < G::NXT) {
@@ -205,12 +219,13 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
 #pragma unroll
       for (int q = 0; q < QZ; ++q) {
         const int z = z0 + q;
-        if (z < ze) {
+        if (!ME || z < ze) {
           R zt = R(0);
 #pragma unroll
           for (int k = 0; k < W; ++k) zt = fmar(win[q + k], tap<R, F>(k), zt);
           const R xc = win[HL + q];
-          const bool has_x = out_ok && coff >= 0 && (zper || (unsigned)(z - zoff) < (unsigned)zlim);
+          const bool has_x =
+              out_ok && coff >= 0 && (zper || (unsigned)(z - zoff) < (unsigned)zlim) && (ME || z < ze);
           R t0 = SX[(q * TY + ty) * SW + tx] * p.a0;
           const R t1 = SY[(q * TY + ty) * SW + tx] * p.a1;
           const R t2 = zt * p.a2;
@@ -222,13 +237,18 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
           if ((p.flags & F_AX) && has_x) t0 = fmar(xc, p.a_x, t0);
           R tot = t1 + t0;
           tot = t2 + tot;
-          if (out_ok) {
-            R* yp = ya + (ocol + zo);
-            if (p.flags & F_AY) tot = fmar(p.a_y, *yp, tot);
-            // streaming store: the output is not read again by this kernel, and
-            // keeping it out of L2 leaves room for the halo values that
-            // neighbouring tiles re-read (5-8% faster at 256^3)
-            __builtin_nontemporal_store(tot, yp);
+          // streaming store: the output is not read again by this kernel, and
+          // keeping it out of L2 leaves room for the halo values that
+          // neighbouring tiles re-read (5-8% faster at 256^3)
+          if constexpr (ME) {
+            if (out_ok) {
+              R* yp = ya + (ocol + zo);
+              if (p.flags & F_AY) tot = fmar(p.a_y, *yp, tot);
+              __builtin_nontemporal_store(tot, yp);
+            }
+          } else {
+            plane_st<R, true>(span_rsrc(ya + zo, z < ze ? syo : 0),
+                              out_ok ? (int)(ocol * (long)sizeof(R)) : kOffNone, tot);
           }
           zo += syo;
         }

@@ -127,6 +127,13 @@ std::string fused_kernel_name(int prec, int form, int id, int v, bool batch) {
     break;
       LC_SUM2W_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PH)                                                       \
+  case I:                                                                                           \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d, %d>",                          \
+             batch ? "sumt_batch" : "sumt_fused", r, filter_tag(id), TX, TY, QX, QY, QZ, WPE, PH);   \
+    break;
+      LC_SUMT_VARIANTS(X)
+#undef X
     }
   } else {
     switch (v) {

@@ -70,6 +70,17 @@ __device__ __forceinline__ R plane_ld(__amdgpu_buffer_rsrc_t rs, int boff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, 0, 0));
 }
 
+// store t at byte offset `boff` of the plane (kOffNone, or a record count
+// of 0: the store is dropped); nt: streaming (non-temporal) store
+template <typename R, bool NT = false>
+__device__ __forceinline__ void plane_st(__amdgpu_buffer_rsrc_t rs, int boff, R t) {
+  typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+  if constexpr (sizeof(R) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, t), rs, boff, 0, NT ? 2 : 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, t), rs, boff, 0, NT ? 2 : 0);
+}
+
 template <typename R>
 __host__ __device__ __forceinline__ int plane_boff(int off) {  // element offset (< 0: none)
   return off >= 0 ? off * (int)sizeof(R) : kOffNone;

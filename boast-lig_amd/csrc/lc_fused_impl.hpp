@@ -17,6 +17,7 @@
 #include "lc_body_sepp.hpp"
 #include "lc_body_sepr.hpp"
 #include "lc_body_sepq.hpp"
+#include "lc_body_sumt.hpp"
 #include "lc_body_sump.hpp"
 
 namespace lc {
@@ -137,8 +138,11 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
   int bx, by, zc, arr;
   block_coords(p.tiles_x, bx, by, zc, arr);
-  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC>(p, bx, by, zc, arr,
-                                        (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc, lds);
+  const long bid = (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc;
+  if (p.flags & F_AY)
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, 1, true>(p, bx, by, zc, arr, bid, lds);
+  else
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, 1, false>(p, bx, by, zc, arr, bid, lds);
 }
 
 template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC>
@@ -151,7 +155,11 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = local % tiles, zc = local / tiles;
   __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
-  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
+  if (p.flags & F_AY)
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, 1, true>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
+  else
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, 1, false>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local,
+                                                         lds);
 }
 
 // the same with the column planes PW > 1 steps in flight
@@ -161,8 +169,11 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
   int bx, by, zc, arr;
   block_coords(p.tiles_x, bx, by, zc, arr);
-  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW>(p, bx, by, zc, arr,
-                                            (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc, lds);
+  const long bid = (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc;
+  if (p.flags & F_AY)
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW, true>(p, bx, by, zc, arr, bid, lds);
+  else
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW, false>(p, bx, by, zc, arr, bid, lds);
 }
 
 template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC, int PW>
@@ -175,7 +186,12 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = local % tiles, zc = local / tiles;
   __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
-  sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
+  if (p.flags & F_AY)
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW, true>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local,
+                                                        lds);
+  else
+    sum3d2_body<R, F, TX, TY, QX, QY, QZ, KC, PW, false>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local,
+                                                         lds);
 }
 
 template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int KC, int ZC, int PW = 1>
@@ -233,6 +249,74 @@ hipError_t launch_sum3d2(const Box3& b, int narr, const R* x, R* y, const double
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !dot) return e;
   return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
+}
+
+// ---- trailing-window kinetic form (lc_body_sumt.hpp)
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int PH>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sumt_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  sumt_body<R, F, TX, TY, QX, QY, QZ, PH>(p, bx, by, zc, arr,
+                                          (bx + (long)p.tiles_x * by) + (long)gridDim.x * zc, lds);
+}
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int PH>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE))) void sumt_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[Sum2Geo<R, F, TX, TY, QX, QY, QZ>::lds_bytes()];
+  sumt_body<R, F, TX, TY, QX, QY, QZ, PH>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, (long)local, lds);
+}
+
+template <class F, int ZC, int Q>
+int sum_zchunk(long tiles, int nz, int narr);
+
+// the trailing window needs a symmetric filter (the queue holds UP planes)
+template <class F>
+constexpr bool sumt_filter() { return -F::LO == F::UP && F::UP % 2 == 0; }
+
+template <typename R, class F, int TX, int TY, int QX, int QY, int QZ, int WPE, int ZC, int PH>
+hipError_t launch_sumt(const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,
+                       double* dot, hipStream_t st, double* scratch, size_t scratch_bytes) {
+  if constexpr (!sumt_filter<F>()) {
+    return hipErrorInvalidValue;
+  } else {
+    if (!plane_fits(b, sizeof(R))) return hipErrorInvalidValue;
+    FusedArgs<R> p{};
+    p.x = x; p.y = y;
+    p.d0 = fdim(b.n[0], b.bc[0], b.nx[0], b.ny[0], F::LO, F::UP);
+    p.d1 = fdim(b.n[1], b.bc[1], b.nx[1], b.ny[1], F::LO, F::UP);
+    p.d2 = fdim(b.n[2], b.bc[2], b.nx[2], b.ny[2], F::LO, F::UP);
+    p.nti = (long)b.nx[0] * b.nx[1] * b.nx[2];
+    p.nto = (long)b.ny[0] * b.ny[1] * b.ny[2];
+    p.narr = narr;
+    p.tiles_x = (p.d0.nout + TX - 1) / TX;
+    p.tiles_y = (p.d1.nout + TY - 1) / TY;
+    const long tiles = (long)p.tiles_x * p.tiles_y;
+    p.zchunk = sum_zchunk<F, ZC, QZ>(tiles, p.d2.nout, narr);
+    const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+    p.a0 = (R)a[0]; p.a1 = (R)a[1]; p.a2 = (R)a[2];
+    p.a_x = a_x; p.a_y = a_y;
+    p.flags = (a_x != R(0) ? F_AX : 0) | (a_y != R(0) ? F_AY : 0) | (dot ? F_DOT : 0);
+    p.nparts = tiles * zch;
+    if (dot) {
+      if (scratch_bytes < sizeof(double) * 3 * (size_t)p.nparts) return hipErrorInvalidValue;
+      p.part = scratch;
+    }
+    if (narr == 0) return hipSuccess;
+    dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+    sumt_fused<R, F, TX, TY, QX, QY, QZ, WPE, PH><<<grid, TX * TY, 0, st>>>(p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !dot) return e;
+    return sum_partial_sets(scratch, p.nparts, dot, 3, p.nparts, st);
+  }
 }
 
 // ---- pipelined kinetic form (lc_body_sump.hpp)
@@ -505,7 +589,10 @@ __attribute__((amdgpu_waves_per_eu(WPE))) void sepq_fused(FusedArgs<R> p) {
   __shared__ __align__(16) unsigned char lds[SepqGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
   int bx, by, zc, arr;
   block_coords(p.tiles_x, bx, by, zc, arr);
-  sepq_body<R, F, TX, TY, QX, QY, PD>(p, bx, by, zc, arr, lds);
+  if (p.flags & (F_AY | F_POT))
+    sepq_body<R, F, TX, TY, QX, QY, PD, true>(p, bx, by, zc, arr, lds);
+  else
+    sepq_body<R, F, TX, TY, QX, QY, PD, false>(p, bx, by, zc, arr, lds);
 }
 
 template <typename R, class F, int TX, int TY, int QX, int QY, int PD, int WPE>
@@ -519,7 +606,10 @@ __attribute__((amdgpu_waves_per_eu(WPE))) void sepq_batch(const FusedArgs<R>* __
   const int tiles = p.tiles_x * p.tiles_y;
   const int tile = local % tiles, zc = local / tiles;
   __shared__ __align__(16) unsigned char lds[SepqGeo<R, F, TX, TY, QX, QY, PD>::lds_bytes()];
-  sepq_body<R, F, TX, TY, QX, QY, PD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+  if (p.flags & (F_AY | F_POT))
+    sepq_body<R, F, TX, TY, QX, QY, PD, true>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+  else
+    sepq_body<R, F, TX, TY, QX, QY, PD, false>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
 }
 
 template <typename R, class F, int TX, int TY, int QX, int QY>
@@ -630,8 +720,15 @@ hipError_t launch_sepq_batch(const FusedArgs<R>* tab, const BatchStage* s, hipSt
 // two-plane kinetic form with the column planes PW steps in flight: as
 // LC_SUM2_VARIANTS, then PW
 #define LC_SUM2W_VARIANTS(X) \
-  X(46, 32, 16, 8, 8, 2, 4, 0, -2, 2) X(47, 32, 16, 8, 8, 2, 2, 0, -2, 2) X(48, 32, 16, 8, 8, 4, 2, 8, -2, 2)
-constexpr int kFusedVariantCount = 49;
+  X(46, 32, 16, 8, 8, 2, 4, 0, -2, 2) X(47, 32, 16, 8, 8, 2, 2, 0, -2, 2) X(48, 32, 16, 8, 8, 4, 2, 8, -2, 2) \
+  X(49, 32, 32, 8, 8, 2, 4, 0, -1, 2) X(50, 32, 32, 8, 8, 2, 4, 0, -1, 1)
+// trailing-window kinetic form (sum only, symmetric filters): tile, QX / QY
+// outputs per task, QZ planes per step, waves per EU, z chunk policy, steps
+// of planes in flight
+#define LC_SUMT_VARIANTS(X) \
+  X(51, 16, 16, 8, 8, 2, 3, -3, 1) X(52, 16, 16, 4, 4, 2, 3, -3, 1) X(53, 32, 16, 8, 8, 2, 2, -2, 1) \
+  X(54, 16, 16, 4, 4, 2, 3, -3, 2) X(55, 32, 16, 8, 8, 2, 2, -2, 2) X(56, 16, 16, 4, 4, 2, 2, -3, 2)
+constexpr int kFusedVariantCount = 57;
 constexpr int kSepVariantCount = 73;
 
 template <typename R, class F>
@@ -657,6 +754,10 @@ hipError_t sum3d_variant(int v, const Box3& b, int narr, const R* x, R* y, const
 #define X(I, TX, TY, QX, QY, QZ, WPE, KC, ZC, PW) \
   case I: return launch_sum3d2<R, F, TX, TY, QX, QY, QZ, WPE, KC, ZC, PW>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
     LC_SUM2W_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PH) \
+  case I: return launch_sumt<R, F, TX, TY, QX, QY, QZ, WPE, ZC, PH>(b, narr, x, y, a, a_x, a_y, dot, st, sc, sb);
+    LC_SUMT_VARIANTS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -781,6 +882,10 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
   case I: return items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
         LC_SUM2W_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PH) \
+  case I: return sumt_filter<F>() && items_fit(it, count) ? fused_stage_build<R, F, TX, TY>(kind, it, count, s) : hipErrorInvalidValue;
+        LC_SUMT_VARIANTS(X)
+#undef X
       }
     } else {
       switch (s->variant) {
@@ -836,6 +941,16 @@ hipError_t fused_stage_t(bool build, int kind, const BatchItem<R>* it, int count
     sum3d2w_batch<R, F, TX, TY, QX, QY, QZ, WPE, KC, PW><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
       LC_SUM2W_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, QY, QZ, WPE, ZC, PH)                                                        \
+  case I:                                                                                            \
+    if constexpr (sumt_filter<F>()) {                                                                \
+      sumt_batch<R, F, TX, TY, QX, QY, QZ, WPE, PH><<<s->blocks, TX * TY, 0, st>>>(tab, s->first, s->count); \
+      break;                                                                                         \
+    } else {                                                                                         \
+      return hipErrorInvalidValue;                                                                   \
+    }
+      LC_SUMT_VARIANTS(X)
 #undef X
       default: return hipErrorInvalidValue;
     }

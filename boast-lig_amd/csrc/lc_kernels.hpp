@@ -57,7 +57,9 @@ struct Box3 {
 // the two-plane bodies read input planes through 32-bit buffer records:
 // planes of at most kPlaneMax bytes
 constexpr long kPlaneMax = 0x7fffffffL;
-inline bool plane_fits(const Box3& b, int prec) { return (long)b.nx[0] * b.nx[1] * prec <= kPlaneMax; }
+inline bool plane_fits(const Box3& b, int prec) {
+  return (long)b.nx[0] * b.nx[1] * prec <= kPlaneMax && (long)b.ny[0] * b.ny[1] * prec <= kPlaneMax;
+}
 // wavelet arrays hold 2 x 2 values per (x, y) step
 inline bool wav_plane_fits(const Box3& b, int prec) { return 4L * b.nx[0] * b.nx[1] * prec <= kPlaneMax; }
 
