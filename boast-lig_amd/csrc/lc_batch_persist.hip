@@ -12,13 +12,13 @@
 // i writes what j reads or writes, or j writes what i reads.  A unit of j
 // waits until done[i] equals i's block count (thread 0 spins, bounded: past
 // the limit it records an error in the status word and goes on, so every
-// wave ends).  Publication (only for items another item waits for, and
-// Kinetic items whose last block sums the energies; the fence writes back
-// the XCD's L2, which costs): every wave waits for its own stores
-// (s_waitcnt), the block synchronises, then one agent-scope fence (L2 write
-// back) precedes the increment of done[item]; the waiting side fences
-// (L2 invalidate) after its wait.  The block that completes a Kinetic item
-// sums the item's energy partials in a fixed order.
+// wave ends).  Publication (only for items another item waits for; the fence
+// writes back the XCD's L2, which costs): every wave waits for its own
+// stores (s_waitcnt), the block synchronises, then one agent-scope fence (L2
+// write back) precedes the increment of done[item]; the waiting side fences
+// (L2 invalidate) after its wait.  The Kinetic items' energies are summed
+// after the launch by one small kernel, one block per item, each over the
+// item's per-block partials in a fixed order.
 //
 // Progress: all 8 k blocks are resident (k from the occupancy query), every
 // queue is ordered by depth, and depth-0 units never wait; by induction on
@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "lc_body_conv.hpp"
+#include "lc_body_sepq.hpp"
 #include "lc_body_sum2.hpp"
 #include "lc_body_wav.hpp"
 
@@ -40,7 +41,7 @@ struct PItem {
   int body;
   int nblocks;
   int dep_off, dep_cnt;
-  int publish;   // a later item waits for it, or its last block sums energies
+  int publish;   // a later item waits for it
   double* dot;   // energies of a Kinetic item (3 doubles) or null
 };
 
@@ -50,17 +51,46 @@ struct PState {  // device; zeroed before every run
   int pad[7];
 };
 
-// tile shapes of the bodies in the persistent kernel (256 lanes each)
-#define LC_P_SEP(R, F) sep3d_body<R, F, 32, 16, 2, 4, 4, 1>
-#define LC_P_SUM(R) sum3d2_body<R, FD2, 16, 16, 4, 4, 2, 0>
-#define LC_P_WAV(R, OP) sepw3d_body<R, OP, 16, 8, 4, 1, 1>
+// The bodies of the persistent kernel, 256 lanes each: the 128-bit-LDS magic
+// filter (FP64 16 x 16, FP32 32 x 16 tiles), the two-plane Kinetic form
+// (16 x 16, two planes per step, four outputs per task; FP64 with the column
+// planes two steps in flight) and the wavelet body on 16 x 8 steps.
+constexpr int kPThreads = 256;
+template <typename R> struct PShape;
+template <> struct PShape<double> {
+  static constexpr int MX = 16, MY = 16, MQX = 4, MQY = 2, KPW = 2, WBZ = 1;
+};
+template <> struct PShape<float> {
+  static constexpr int MX = 32, MY = 16, MQX = 8, MQY = 4, KPW = 1, WBZ = 1;
+};
+template <typename R, class F>
+using PSepGeo = SepqGeo<R, F, PShape<R>::MX, PShape<R>::MY, PShape<R>::MQX, PShape<R>::MQY, 2>;
+template <typename R>
+using PSumGeo = Sum2Geo<R, FD2, 16, 16, 4, 4, 2>;
+
+template <typename R, class F, bool ME>
+__device__ __forceinline__ void p_sep(const FusedArgs<R>& p, int bx, int by, int zc, unsigned char* lds) {
+  using S = PShape<R>;
+  // streaming stores: little for the publishing fence to write back
+  sepq_body<R, F, S::MX, S::MY, S::MQX, S::MQY, 2, ME, true>(p, bx, by, zc, 0, lds);
+}
+template <typename R, bool ME>
+__device__ __forceinline__ void p_sum(const FusedArgs<R>& p, int bx, int by, int zc, long bid, unsigned char* lds) {
+  sum3d2_body<R, FD2, 16, 16, 4, 4, 2, 0, PShape<R>::KPW, ME>(p, bx, by, zc, 0, bid, lds);
+}
+template <typename R, class OP>
+__device__ __forceinline__ void p_wav(const FusedArgs<R>& p, int bx, int by, int zc, unsigned char* lds) {
+  sepw3d_body<R, OP, 16, 8, 4, PShape<R>::WBZ, PShape<R>::WBZ>(p, bx, by, zc, 0, lds);
+}
 
 template <typename R>
 constexpr size_t persist_lds_bytes() {
-  return std::max(std::max(sep3d_lds_bytes<R, FMD, 32, 16, 2, 4, 4, 1>(),
-                           sep3d_lds_bytes<R, FIMD, 32, 16, 2, 4, 4, 1>()),
-                  std::max(Sum2Geo<R, FD2, 16, 16, 4, 4, 2>::lds_bytes(),
-                           sepw3d_lds_bytes<R, Dwt<Sym8Dec>, 16, 8, 4, 1, 1>()));
+  static_assert(PSepGeo<R, FMD>::NT == kPThreads && PSepGeo<R, FIMD>::NT == kPThreads &&
+                    PSumGeo<R>::NT == kPThreads && 2 * 16 * 8 == kPThreads,
+                "every body of the persistent kernel takes kPThreads lanes");
+  return std::max(std::max(PSepGeo<R, FMD>::lds_bytes(), PSepGeo<R, FIMD>::lds_bytes()),
+                  std::max(PSumGeo<R>::lds_bytes(),
+                           sepw3d_lds_bytes<R, Dwt<Sym8Dec>, 16, 8, 4, PShape<R>::WBZ, PShape<R>::WBZ>()));
 }
 
 __global__ void persist_reset(PState* st, int* done, int nitems) {
@@ -69,18 +99,17 @@ __global__ void persist_reset(PState* st, int* done, int nitems) {
   if (i < nitems) done[i] = 0;
 }
 
+// two waves per SIMD (two blocks per CU): the wavelet body takes more than
+// 128 VGPRs (capped there it spills)
 template <typename R>
-// two waves per SIMD at least: the bodies' union would otherwise take more
-// than 256 VGPRs in FP64 (one block per CU)
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_persistent(const FusedArgs<R>* __restrict__ tab,
+__global__ __launch_bounds__(kPThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_persistent(const FusedArgs<R>* __restrict__ tab,
                                                         const PItem* __restrict__ items,
                                                         const int* __restrict__ deps,
                                                         const int2* __restrict__ work,
                                                         const int* __restrict__ qoff, PState* st,
                                                         int* done, int spin_limit) {
   __shared__ __align__(16) unsigned char lds[persist_lds_bytes<R>()];
-  __shared__ int s_pos, s_last;
-  __shared__ double s_red[4];
+  __shared__ int s_pos;
   const int t = threadIdx.x;
   const int queue = blockIdx.x & 7;
   const int qbeg = qoff[queue], qend = qoff[queue + 1];
@@ -113,12 +142,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     const int tiles = p.tiles_x * p.tiles_y;
     const int tile = local % tiles, zc = local / tiles;
     const int bx = tile % p.tiles_x, by = tile / p.tiles_x;
+    const bool me = (p.flags & (F_AY | F_POT)) != 0;  // uniform
     switch (pi.body) {
-      case PB_SEP_MD: LC_P_SEP(R, FMD)(p, bx, by, zc, 0, lds); break;
-      case PB_SEP_IMD: LC_P_SEP(R, FIMD)(p, bx, by, zc, 0, lds); break;
-      case PB_SUM_D2: LC_P_SUM(R)(p, bx, by, zc, 0, (long)local, lds); break;
-      case PB_DWT: LC_P_WAV(R, Dwt<Sym8Dec>)(p, bx, by, zc, 0, lds); break;
-      case PB_IDWT: LC_P_WAV(R, Idwt<Sym8Rec>)(p, bx, by, zc, 0, lds); break;
+      case PB_SEP_MD:
+        if (me) p_sep<R, FMD, true>(p, bx, by, zc, lds); else p_sep<R, FMD, false>(p, bx, by, zc, lds);
+        break;
+      case PB_SEP_IMD:
+        if (me) p_sep<R, FIMD, true>(p, bx, by, zc, lds); else p_sep<R, FIMD, false>(p, bx, by, zc, lds);
+        break;
+      case PB_SUM_D2:
+        if (me) p_sum<R, true>(p, bx, by, zc, (long)local, lds); else p_sum<R, false>(p, bx, by, zc, (long)local, lds);
+        break;
+      case PB_DWT: p_wav<R, Dwt<Sym8Dec>>(p, bx, by, zc, lds); break;
+      case PB_IDWT: p_wav<R, Idwt<Sym8Rec>>(p, bx, by, zc, lds); break;
     }
     // publish: this wave's stores complete, then the block, then one fence.
     // The fence writes back the XCD's L2, so units nobody waits for skip it
@@ -128,24 +164,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     __syncthreads();
     if (t == 0) {
       __threadfence();  // release
-      const int prev = atomicAdd(&done[it], 1);
-      s_last = prev == pi.nblocks - 1;
-      if (s_last) __threadfence();  // acquire: every block's energy partials
+      atomicAdd(&done[it], 1);
     }
-    __syncthreads();
-    if (s_last && pi.body == PB_SUM_D2 && pi.dot) {
-      const long nb = pi.nblocks;
-      for (int k = 0; k < 3; ++k) {
-        double v = 0.0;
-        for (long i = t; i < nb; i += 256) v += p.part[k * nb + i];
+  }
+}
+
+// The energies of the Kinetic items, after the persistent launch: one block
+// per item sums its per-block partials in a fixed order.
+struct PEnergy {
+  const double* part;  // 3 sets of nb partials
+  long nb;
+  double* dot;
+};
+
+__global__ __launch_bounds__(256) void persist_energies(const PEnergy* __restrict__ en) {
+  __shared__ double s_red[4];
+  const PEnergy e = en[blockIdx.x];
+  const int t = threadIdx.x;
+  for (int k = 0; k < 3; ++k) {
+    double v = 0.0;
+    for (long i = t; i < e.nb; i += 256) v += e.part[k * e.nb + i];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((t & 63) == 0) s_red[t >> 6] = v;
-        __syncthreads();
-        if (t == 0) pi.dot[k] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
-        __syncthreads();
-      }
-    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((t & 63) == 0) s_red[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) e.dot[k] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    __syncthreads();
   }
 }
 
@@ -161,12 +205,14 @@ struct PersistPlan {
   PState* st = nullptr;
   int* done = nullptr;
   double* part = nullptr;
+  PEnergy* en = nullptr;  // the Kinetic items with energies
+  int nen = 0;
 };
 
 void persist_free(PersistPlan* pp) {
   if (!pp) return;
   for (void* q : {pp->tab, (void*)pp->items, (void*)pp->deps, (void*)pp->work, (void*)pp->qoff,
-                  (void*)pp->st, (void*)pp->done, (void*)pp->part})
+                  (void*)pp->st, (void*)pp->done, (void*)pp->part, (void*)pp->en})
     if (q) (void)hipFree(q);
   delete pp;
 }
@@ -232,14 +278,14 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
     const int k = kinds[i], id = ids[i];
     PItem& pi = items[i];
     pi.dot = nullptr;
-    if (k == BK_SEP && (id == OP_MD || id == OP_IMD)) {
+    if (k == BK_SEP && (id == OP_MD || id == OP_IMD) && plane_fits(it[i].b, sizeof(R))) {
+      constexpr int MX = PShape<R>::MX, MY = PShape<R>::MY;
       pi.body = id == OP_MD ? PB_SEP_MD : PB_SEP_IMD;
-      tab[i] = id == OP_MD ? conv_args<R, FMD, 32, 16>(it[i], false) : conv_args<R, FIMD, 32, 16>(it[i], false);
+      tab[i] = id == OP_MD ? conv_args<R, FMD, MX, MY>(it[i], false) : conv_args<R, FIMD, MX, MY>(it[i], false);
     } else if (k == BK_SUM && id == OP_D2 && plane_fits(it[i].b, sizeof(R))) {
       pi.body = PB_SUM_D2;
       tab[i] = conv_args<R, FD2, 16, 16>(it[i], true);
       pi.dot = it[i].dot;
-      pi.publish = pi.dot != nullptr;  // its last block sums the energies
     } else if ((k == BK_DWT || k == BK_IDWT) && id == WAV_SYM8 && wav_plane_fits(it[i].b, sizeof(R))) {
       pi.body = k == BK_DWT ? PB_DWT : PB_IDWT;
       tab[i] = k == BK_DWT ? wav_args_p<R, Dwt<Sym8Dec>>(it[i]) : wav_args_p<R, Idwt<Sym8Rec>>(it[i]);
@@ -295,12 +341,16 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
   hipError_t e = hipSuccess;
   if (total_parts > 0) e = hipMalloc(reinterpret_cast<void**>(&pp->part), sizeof(double) * total_parts);
   long off = 0;
+  std::vector<PEnergy> en;
   for (int i = 0; i < count && e == hipSuccess; ++i)
     if (items[i].body == PB_SUM_D2 && items[i].dot) {
       tab[i].part = pp->part + off;
       tab[i].nparts = items[i].nblocks;
+      en.push_back({pp->part + off, (long)items[i].nblocks, items[i].dot});
       off += 3L * items[i].nblocks;
     }
+  pp->nen = (int)en.size();
+  if (e == hipSuccess && pp->nen) e = upload(&pp->en, en);
   if (e == hipSuccess) e = upload(reinterpret_cast<FusedArgs<R>**>(&pp->tab), tab);
   if (e == hipSuccess) e = upload(&pp->items, items);
   if (e == hipSuccess) e = upload(&pp->deps, deps);
@@ -310,7 +360,7 @@ hipError_t persist_build(const BatchItem<R>* it, const int* kinds, const int* id
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&pp->done), sizeof(int) * std::max(count, 1));
   int per_cu = 0;
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_persistent<R>, 256, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_persistent<R>, kPThreads, 0);
   if (e == hipSuccess && per_cu < 1) e = hipErrorInvalidConfiguration;
   if (e != hipSuccess) {
     persist_free(pp);
@@ -328,11 +378,16 @@ hipError_t persist_launch(const PersistPlan* pp, hipStream_t st, int spin_limit)
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || pp->units == 0) return e;
   if (pp->prec == 8)
-    batch_persistent<double><<<pp->grid, 256, 0, st>>>(static_cast<const FusedArgs<double>*>(pp->tab), pp->items,
-                                                       pp->deps, pp->work, pp->qoff, pp->st, pp->done, spin_limit);
+    batch_persistent<double><<<pp->grid, kPThreads, 0, st>>>(static_cast<const FusedArgs<double>*>(pp->tab),
+                                                             pp->items, pp->deps, pp->work, pp->qoff, pp->st,
+                                                             pp->done, spin_limit);
   else
-    batch_persistent<float><<<pp->grid, 256, 0, st>>>(static_cast<const FusedArgs<float>*>(pp->tab), pp->items,
-                                                      pp->deps, pp->work, pp->qoff, pp->st, pp->done, spin_limit);
+    batch_persistent<float><<<pp->grid, kPThreads, 0, st>>>(static_cast<const FusedArgs<float>*>(pp->tab),
+                                                            pp->items, pp->deps, pp->work, pp->qoff, pp->st,
+                                                            pp->done, spin_limit);
+  e = hipGetLastError();
+  if (e != hipSuccess || pp->nen == 0) return e;
+  persist_energies<<<pp->nen, 256, 0, st>>>(pp->en);
   return hipGetLastError();
 }
 

@@ -87,7 +87,8 @@ struct SepqGeo {
 // as buffer stores that drop the x-waves' and the invalid lanes' values (out
 // of range).  The compiler's wait counts are then exact, and a step waits for
 // the tile loads it uses, not for the previous step's stores.
-template <typename R, class F, int TX, int TY, int QX, int QY, int PD, bool ME = true>
+// NTS (ME false): streaming (non-temporal) output stores.
+template <typename R, class F, int TX, int TY, int QX, int QY, int PD, bool ME = true, bool NTS = false>
 __device__ __forceinline__ void sepq_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
                                           unsigned char* lds) {
   using G = SepqGeo<R, F, TX, TY, QX, QY, PD>;
@@ -252,7 +253,7 @@ __device__ __forceinline__ void sepq_body(const FusedArgs<R>& p, int bx, int by,
         const __amdgpu_buffer_rsrc_t ro = span_rsrc(ya + zo, em ? syo : 0);
 #pragma unroll
         for (int i = 0; i < QY; ++i)
-          plane_st<R>(ro, (!xwave && xin && i < yrows) ? (int)((ocol + (long)i * p.d0.ny) * (long)sizeof(R)) : kOffNone,
+          plane_st<R, NTS>(ro, (!xwave && xin && i < yrows) ? (int)((ocol + (long)i * p.d0.ny) * (long)sizeof(R)) : kOffNone,
                       outv[i]);
         zo += em ? syo : 0;
       }
