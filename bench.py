@@ -387,16 +387,38 @@ def run_batch(torch, lc, dtype, reps, count=512):
                      'MF3D -> K3D with energies, and a sym8 DWT of (2s)^3 values'}
 
 
+# the sources that define the chain kernels' code: a counter summary taken
+# from other sources describes other code
+KERNEL_SOURCES = ['lc_common.hpp', 'lc_taps.h', 'lc_kernels.hpp', 'lc_fused.hpp', 'lc_body_conv.hpp',
+                  'lc_body_sepq.hpp', 'lc_body_sum2.hpp', 'lc_fused_impl.hpp']
+
+
+def kernel_source_digest():
+    """SHA-256 (first 16 hex digits) of the chain kernels' sources and build flags."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES + ['../Makefile']:
+        with open(os.path.join(ROOT, 'boast-lig_amd', 'csrc', name), 'rb') as f:
+            h.update(name.encode() + b'\0' + f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel):
-    """Bytes per launch of `kernel` from the committed counter summary, or None."""
+    """Bytes per launch of `kernel` from the committed counter summary, or None
+    when the summary lacks the kernel or was taken from other kernel sources
+    (its `source_digest` line differs from kernel_source_digest())."""
     import re
     if not kernel or not os.path.exists(TRAFFIC_FILE):
         return None
+    digest, found = None, None
     for line in open(TRAFFIC_FILE):
+        m = re.match(r'source_digest = (\w+)', line.strip())
+        if m:
+            digest = m.group(1)
         m = re.match(r'traffic\[(.*)\] = (\d+)', line.strip())
         if m and m.group(1) == kernel:
-            return int(m.group(2))
-    return None
+            found = int(m.group(2))
+    return found if digest == kernel_source_digest() else None
 
 
 def errs(a, b):
@@ -522,7 +544,12 @@ def main():
     ap.add_argument('--no-extra', action='store_true', help='skip the wavelet and batch lines')
     ap.add_argument('--copy-calib', action='store_true',
                     help='only run a 1 GiB device copy (calibrates the HBM byte counters)')
+    ap.add_argument('--source-digest', action='store_true',
+                    help='print the digest of the chain kernels\' sources (for the counter summary)')
     args = ap.parse_args()
+    if args.source_digest:
+        print(kernel_source_digest())
+        return 0
     if args.copy_calib:
         import torch
         nb = 1 << 30
