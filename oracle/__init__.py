@@ -50,9 +50,13 @@ def lib():
         L.lco_div_u_i.argtypes = [i, ip, dp, dp, i, dp, dp]
         L.lco_nabla_u.argtypes = [i, ip, dp, dp, i, dp]
         L.lco_nabla_u_and_square.argtypes = [i, ip, dp, dp, dp, i, dp]
+        L.lco_nabla_u_square.argtypes = [i, ip, dp, dp, i, dp]
+        L.lco_nabla_u_epsilon.argtypes = [i, ip, dp, dp, i, dp, dp]
+        L.lco_update_rhopol.argtypes = [i, ip, dp, i, dp, d, dp, dp, dp]
         for f in ('lco_s0s0_1d', 'lco_s0s1_1d', 'lco_s1s0_1d', 'lco_s0s0_md', 'lco_s0s0_1ds',
                   'lco_s0s1_md', 'lco_s1s0_md', 'lco_filter_taps', 'lco_num_threads', 'lco_poisson_taps',
-                  'lco_poisson_1d', 'lco_div_u_i', 'lco_nabla_u', 'lco_nabla_u_and_square'):
+                  'lco_poisson_1d', 'lco_div_u_i', 'lco_nabla_u', 'lco_nabla_u_and_square',
+                  'lco_nabla_u_square', 'lco_nabla_u_epsilon', 'lco_update_rhopol'):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -237,6 +241,29 @@ def nabla_u_and_square(geocode, n, u, du, du2, nord, h):
     hv = np.ascontiguousarray(np.asarray(h, dtype=np.float64))
     _check(lib().lco_nabla_u_and_square(geocode, pn, _darr(u), _darr(du), _darr(du2), nord, _darr(hv)),
            'nabla_u_and_square')
+
+
+def nabla_u_square(geocode, n, u, ddu, nord, h):
+    n_, pn = _iarr(n)
+    hv = np.ascontiguousarray(np.asarray(h, dtype=np.float64))
+    _check(lib().lco_nabla_u_square(geocode, pn, _darr(u), _darr(ddu), nord, _darr(hv)), 'nabla_u_square')
+
+
+def nabla_u_epsilon(geocode, n, u, du, nord, h, eps):
+    n_, pn = _iarr(n)
+    hv = np.ascontiguousarray(np.asarray(h, dtype=np.float64))
+    _check(lib().lco_nabla_u_epsilon(geocode, pn, _darr(u), _darr(du), nord, _darr(hv), _darr(eps)),
+           'nabla_u_epsilon')
+
+
+def update_rhopol(geocode, n, u, nord, h, eta, dlogeps, rhopol):
+    """rhopol updated in place; returns rhores2."""
+    n_, pn = _iarr(n)
+    hv = np.ascontiguousarray(np.asarray(h, dtype=np.float64))
+    r = np.zeros(1)
+    _check(lib().lco_update_rhopol(geocode, pn, _darr(u), nord, _darr(hv), float(eta), _darr(dlogeps),
+                                   _darr(rhopol), _darr(r)), 'update_rhopol')
+    return float(r[0])
 
 
 # ---------------------------------------------------------------------------

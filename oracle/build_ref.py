@@ -18,8 +18,10 @@ a trailing underscore, every argument by reference):
   libref_wav.so   synthesis_{free,per}_ref,           BigDFT/old/Synthesis.rb:6-147
                   analysis_{free,per}_ref             BigDFT/old/AnaRotPer-2.rb:6-251
   libref_poisson.so generate_filter, div_u_i_ref,     BigDFT/Poisson2.rb (heredocs)
-                  div_u_i_opt, nabla_u_ref (nabla_u_and_square_ref
-                  needs BigDFT's f_err_throw: not built)
+                  div_u_i_opt, nabla_u_ref, nabla_u_square_ref,
+                  nabla_u_epsilon_ref, update_rhopol_ref
+                  (nabla_u_and_square_ref needs BigDFT's f_err_throw:
+                  not built)
 """
 import argparse
 import json
@@ -118,7 +120,8 @@ def main():
         # nabla_u_and_square_ref calls BigDFT's f_err_throw, which the
         # reference does not carry: that routine is not built here
         compile_so([('poi_%s.f90' % f, heredoc_after(poi, 'function_name = "%s"' % f))
-                    for f in ('generate_filter', 'div_u_i_ref', 'div_u_i_opt', 'nabla_u_ref')],
+                    for f in ('generate_filter', 'div_u_i_ref', 'div_u_i_opt', 'nabla_u_ref',
+                              'nabla_u_square_ref', 'nabla_u_epsilon_ref', 'update_rhopol_ref')],
                    os.path.join(args.out, 'libref_poisson.so'), tmp)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)

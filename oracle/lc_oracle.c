@@ -709,3 +709,78 @@ int lco_nabla_u_and_square(int geocode, const int *n, const double *u, double *d
     }
     return 0;
 }
+
+/* nabla_u_square (Poisson2.rb:2278-2359, nabla_u_square_ref :1690-1884):
+ * ddu = (d_x^2 + d_y^2) + d_z^2, the gradient itself not kept */
+int lco_nabla_u_square(int geocode, const int *n, const double *u, double *ddu, int nord,
+                       const double *h) {
+    if (!poisson_dims_ok(nord, n) || !h) return -1;
+    const double *tab = poisson_table(nord);
+    const int m = nord / 2;
+    int per[3];
+    geo_per(geocode, per);
+    const long np = (long)n[0] * n[1] * n[2];
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < np; ++p) {
+        double g[3];
+        grad_at(tab, m, per, n, u, p, h, 7, g);
+        double t = g[0] * g[0];
+        t = t + g[1] * g[1];
+        ddu[p] = t + g[2] * g[2];
+    }
+    return 0;
+}
+
+/* nabla_u_epsilon (Poisson2.rb:2362-2443, nabla_u_epsilon_ref :1488-1680):
+ * du(:, :, :, k) = (D_k u / h_k) eps */
+int lco_nabla_u_epsilon(int geocode, const int *n, const double *u, double *du, int nord,
+                        const double *h, const double *eps) {
+    if (!poisson_dims_ok(nord, n) || !h || !eps) return -1;
+    const double *tab = poisson_table(nord);
+    const int m = nord / 2;
+    int per[3];
+    geo_per(geocode, per);
+    const long np = (long)n[0] * n[1] * n[2];
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < np; ++p) {
+        double g[3];
+        grad_at(tab, m, per, n, u, p, h, 7, g);
+        for (int k = 0; k < 3; ++k) du[k * np + p] = g[k] * eps[p];
+    }
+    return 0;
+}
+
+/* update_rhopol (Poisson2.rb:2446-2550, update_rhopol_ref :520-699): per
+ * point res = ((e1 d_x + e2 d_y) + e3 d_z) / (4 pi) with (e1, e2, e3) =
+ * dlogeps(:, p); res = eta (res - rhopol); rhopol = res + rhopol; rhores2 =
+ * the sum of res^2 over the points in storage order */
+int lco_update_rhopol(int geocode, const int *n, const double *u, int nord, const double *h,
+                      double eta, const double *dlogeps, double *rhopol, double *rhores2) {
+    if (!poisson_dims_ok(nord, n) || !h || !dlogeps || !rhopol || !rhores2) return -1;
+    const double *tab = poisson_table(nord);
+    const int m = nord / 2;
+    const double oneo4pi = 0.07957747154594766788444188168625718101722982287022822437383367203;
+    int per[3];
+    geo_per(geocode, per);
+    const long np = (long)n[0] * n[1] * n[2];
+    double *sq = (double *)malloc((size_t)(np > 0 ? np : 1) * sizeof(double));
+    if (!sq) return -1;
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < np; ++p) {
+        double g[3];
+        grad_at(tab, m, per, n, u, p, h, 7, g);
+        const double *e = dlogeps + 3 * p;
+        double res = e[0] * g[0] + e[1] * g[1];
+        res = res + e[2] * g[2];
+        res = res * oneo4pi;
+        const double rho = rhopol[p];
+        res = (res - rho) * eta;
+        sq[p] = res * res;
+        rhopol[p] = res + rho;
+    }
+    double s = 0.0;
+    for (long p = 0; p < np; ++p) s += sq[p];
+    free(sq);
+    *rhores2 = s;
+    return 0;
+}

@@ -94,6 +94,17 @@ int lco_nabla_u(int geocode, const int *n, const double *u, double *du, int nord
                 const double *h);
 int lco_nabla_u_and_square(int geocode, const int *n, const double *u, double *du,
                            double *du2, int nord, const double *h);
+/* ddu = |grad u|^2 only */
+int lco_nabla_u_square(int geocode, const int *n, const double *u, double *ddu, int nord,
+                       const double *h);
+/* du(:, :, :, k) = eps D_k u / h_k */
+int lco_nabla_u_epsilon(int geocode, const int *n, const double *u, double *du, int nord,
+                        const double *h, const double *eps);
+/* res = (dlogeps . grad u) / (4 pi); res = eta (res - rhopol);
+ * rhopol += res; *rhores2 = sum of res^2 (points in storage order);
+ * dlogeps (3, n0, n1, n2), the component fastest */
+int lco_update_rhopol(int geocode, const int *n, const double *u, int nord, const double *h,
+                      double eta, const double *dlogeps, double *rhopol, double *rhores2);
 
 #ifdef __cplusplus
 }

@@ -130,3 +130,35 @@ def nabla_u(geocode, n, u, nord, h, table):
     L.nabla_u_ref_(_i(geocode), _i(n[0]), _i(n[1]), _i(n[2]), _p(u), _p(du), _i(nord), _p(hv),
                    _p(_c1d(table)))
     return du
+
+
+def nabla_u_square(geocode, n, u, nord, h, table):
+    """nabla_u_square_ref (BigDFT/Poisson2.rb:1690-1884): du2 (n0, n1, n2)."""
+    L = _load('libref_poisson.so')
+    hv = np.asfortranarray(np.asarray(h, dtype=np.float64))
+    du2 = np.zeros(tuple(n), order='F')
+    L.nabla_u_square_ref_(_i(geocode), _i(n[0]), _i(n[1]), _i(n[2]), _p(u), _p(du2), _i(nord), _p(hv),
+                          _p(_c1d(table)))
+    return du2
+
+
+def nabla_u_epsilon(geocode, n, u, nord, h, eps, table):
+    """nabla_u_epsilon_ref (BigDFT/Poisson2.rb:1488-1680): du (n0, n1, n2, 3)."""
+    L = _load('libref_poisson.so')
+    hv = np.asfortranarray(np.asarray(h, dtype=np.float64))
+    du = np.zeros(tuple(n) + (3,), order='F')
+    L.nabla_u_epsilon_ref_(_i(geocode), _i(n[0]), _i(n[1]), _i(n[2]), _p(u), _p(du), _i(nord), _p(hv),
+                           _p(eps), _p(_c1d(table)))
+    return du
+
+
+def update_rhopol(geocode, n, u, nord, h, eta, dlogeps, rhopol, table):
+    """update_rhopol_ref (BigDFT/Poisson2.rb:520-699): the updated rhopol
+    (n0, n1, n2) and rhores2; dlogeps is (3, n0, n1, n2) Fortran order."""
+    L = _load('libref_poisson.so')
+    hv = np.asfortranarray(np.asarray(h, dtype=np.float64))
+    rp = np.array(rhopol, dtype=np.float64, order='F', copy=True)
+    r2 = np.zeros(1)
+    L.update_rhopol_ref_(_i(geocode), _i(n[0]), _i(n[1]), _i(n[2]), _p(u), _i(nord), _p(hv), _d(eta),
+                         _p(dlogeps), _p(rp), _p(r2), _p(_c1d(table)))
+    return rp, float(r2[0])

@@ -16,6 +16,8 @@ that the reference's FP64 code produced for them:
                                                         BigDFT/old/AnaRotPer-2.rb
   poisson.npz generate_filter tables (nord 2..16), div_u_i_ref, div_u_i_opt
               (du, cc), nabla_u_ref                     BigDFT/Poisson2.rb
+  poisson_dielectric.npz  nabla_u_square_ref, nabla_u_epsilon_ref,
+              update_rhopol_ref (rhopol and rhores2)    BigDFT/Poisson2.rb
 
 Usage: python3 tests/golden/make_golden.py  (from the repository root).
 """
@@ -125,6 +127,37 @@ def poisson():
     np.savez_compressed(os.path.join(OUT, 'poisson.npz'), **out)
 
 
+def poisson_dielectric():
+    """nabla_u_square_ref, nabla_u_epsilon_ref and update_rhopol_ref (eta
+    0.5, as Poisson_test.rb runs it; rhopol starts from a seeded field, not
+    zeros, so that its update is tested too) on seeded fields, each with the
+    reference's own table, and for nord 8 with the exact Lagrange table."""
+    from oracle import poisson_taps
+    rng = np.random.default_rng(606)
+    h = np.array([0.15, 0.2, 0.25])
+    out = dict(h=h, eta=0.5)
+    for key, n in (('small', (11, 12, 13)), ('big', (17, 17, 17))):
+        out['n_' + key] = np.array(n)
+        out['u_' + key] = np.asfortranarray(rng.uniform(-1, 1, n))
+        out['eps_' + key] = np.asfortranarray(rng.uniform(0.5, 2.0, n))
+        out['dlogeps_' + key] = np.asfortranarray(rng.uniform(-1, 1, (3,) + n))
+        out['rhopol_' + key] = np.asfortranarray(rng.uniform(-1, 1, n))
+    cases = [(2, 1, 'ref'), (4, 3, 'ref'), (6, 2, 'ref'), (8, 1, 'lagrange'), (16, 3, 'ref')]
+    for nord, geo, tab in cases:
+        key = 'big' if nord == 16 else 'small'
+        n = tuple(int(v) for v in out['n_' + key])
+        u = out['u_' + key]
+        table = R.poisson_filter(nord) if tab == 'ref' else poisson_taps(nord)
+        c = '%d_%d_%s' % (nord, geo, tab)
+        out['square_' + c] = R.nabla_u_square(geo, n, u, nord, h, table)
+        out['epsilon_' + c] = R.nabla_u_epsilon(geo, n, u, nord, h, out['eps_' + key], table)
+        rp, r2 = R.update_rhopol(geo, n, u, nord, h, out['eta'], out['dlogeps_' + key],
+                                 out['rhopol_' + key], table)
+        out['rhopol_' + c], out['rhores2_' + c] = rp, r2
+    out['cases'] = np.array(['%d_%d_%s' % c for c in cases])
+    np.savez_compressed(os.path.join(OUT, 'poisson_dielectric.npz'), **out)
+
+
 def main():
     for lib in ('libref_mf.so', 'libref_kin.so', 'libref_idwt.so', 'libref_wav.so', 'libref_poisson.so'):
         if not R.available(lib):
@@ -135,6 +168,7 @@ def main():
     idwt()
     bigdft_wavelet()
     poisson()
+    poisson_dielectric()
     print('make_golden: wrote', sorted(f for f in os.listdir(OUT) if f.endswith('.npz')))
     return 0
 

@@ -611,6 +611,81 @@ def test_poisson_oracle_matches_reference_routines(golden_dir):
         assert rel(gr, ref_gr.reshape(-1)) < 1e-13, case
 
 
+def test_poisson_dielectric_oracle_matches_reference_routines(golden_dir):
+    """nabla_u_square, nabla_u_epsilon and update_rhopol of the oracle against
+    nabla_u_square_ref, nabla_u_epsilon_ref and update_rhopol_ref run on the
+    same fields (tests/golden/poisson_dielectric.npz)."""
+    g = np.load(os.path.join(golden_dir, 'poisson_dielectric.npz'))
+    h, eta = g['h'], float(g['eta'])
+    for case in g['cases']:
+        c = str(case)
+        nord, geo, _ = c.split('_')
+        nord, geo = int(nord), int(geo)
+        key = 'big' if nord == 16 else 'small'
+        n = [int(v) for v in g['n_' + key]]
+        f = lambda a: np.ascontiguousarray(a.reshape(-1, order='F'))  # noqa: E731
+        u, eps, dl, rp = f(g['u_' + key]), f(g['eps_' + key]), f(g['dlogeps_' + key]), f(g['rhopol_' + key])
+        np_ = u.size
+        sq = np.zeros(np_)
+        O.nabla_u_square(geo, n, u, sq, nord, h)
+        assert rel(sq, f(g['square_' + c])) < 1e-13, c
+        de = np.zeros(3 * np_)
+        O.nabla_u_epsilon(geo, n, u, de, nord, h, eps)
+        assert rel(de, f(g['epsilon_' + c])) < 1e-13, c
+        r2 = O.update_rhopol(geo, n, u, nord, h, eta, dl, rp)
+        assert rel(rp, f(g['rhopol_' + c])) < 1e-13, c
+        assert abs(r2 - float(g['rhores2_' + c])) <= 1e-13 * abs(r2), c
+
+
+@pytest.mark.parametrize('geo', [1, 2, 3])
+def test_poisson_dielectric_identities(geo):
+    """Closed forms: the square is |nabla_u|^2 summed in x, y, z order; the
+    epsilon form is nabla_u scaled pointwise; update_rhopol with eta = 0
+    makes every res 0, so rhopol is unchanged and rhores2 is 0; with eta = 1
+    and rhopol = 0 the new rhopol is the projected gradient over 4 pi and
+    rhores2 its square sum; on u linear along a free x every order
+    differentiates exactly, so rhopol = slope / (4 pi) everywhere."""
+    rng = np.random.default_rng(700 + geo)
+    n = [12, 11, 10]
+    h = [0.3, 0.2, 0.1]
+    np_ = int(np.prod(n))
+    u = rng.uniform(-1, 1, np_)
+    for nord in O.POISSON_ORDERS[:4]:
+        gr = np.zeros(3 * np_)
+        O.nabla_u(geo, n, u, gr, nord, h)
+        gx, gy, gz = gr[:np_], gr[np_:2 * np_], gr[2 * np_:]
+        sq = np.zeros(np_)
+        O.nabla_u_square(geo, n, u, sq, nord, h)
+        assert np.array_equal(sq, (gx * gx + gy * gy) + gz * gz)
+        eps = rng.uniform(0.5, 2, np_)
+        de = np.zeros(3 * np_)
+        O.nabla_u_epsilon(geo, n, u, de, nord, h, eps)
+        assert np.array_equal(de, gr * np.tile(eps, 3))
+        dl = rng.uniform(-1, 1, 3 * np_)
+        rp0 = rng.uniform(-1, 1, np_)
+        rp = rp0.copy()
+        assert O.update_rhopol(geo, n, u, nord, h, 0.0, dl, rp) == 0.0
+        assert np.array_equal(rp, rp0)
+        rp = np.zeros(np_)
+        r2 = O.update_rhopol(geo, n, u, nord, h, 1.0, dl, rp)
+        e = dl.reshape(np_, 3)
+        want = ((e[:, 0] * gx + e[:, 1] * gy) + e[:, 2] * gz) / (4 * np.pi)
+        assert np.abs(rp - want).max() <= 1e-15 * np.abs(want).max() * 8
+        assert abs(r2 - float(want @ want)) <= 1e-12 * r2
+    # a linear field along x on a free x: every order differentiates it exactly
+    if geo == 1:
+        i0 = np.tile(np.arange(n[0], dtype=np.float64), n[1] * n[2])
+        lin = 3.0 * i0 * h[0]
+        dl = np.tile([1.0, 0.0, 0.0], np_)
+        rp = np.zeros(np_)
+        for nord in O.POISSON_ORDERS:
+            if min(n) <= nord:
+                continue
+            rp[:] = 0.0
+            O.update_rhopol(geo, n, lin, nord, h, 1.0, dl, rp)
+            assert np.abs(rp - 3.0 / (4 * np.pi)).max() < 1e-12
+
+
 def test_poisson_broker_and_plane_waves():
     """The 1D operator (Poisson_broker) is one component of nabla_u; on a
     periodic line it maps the plane wave e^{i theta j} to
