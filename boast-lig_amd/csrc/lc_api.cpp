@@ -689,6 +689,62 @@ int run_poisson(int op, int nord, int idim, int bc, int geocode, const int* n, c
   return LC_OK;
 }
 
+enum DielOp { PD_SQUARE, PD_EPS, PD_RHOPOL };
+
+// nabla_u_square (out: ddu), nabla_u_epsilon (in1: eps; out: du, 3 n) and
+// update_rhopol (in1: dlogeps, 3 n; out: rhopol, in and out; rhores2); arrays
+// host (staged, synchronous) or device (device_api: asynchronous, rhores2 a
+// device FP64 value)
+template <typename R>
+int run_dielectric(int op, int geocode, const int* n, const R* u, const R* in1, R* out, int nord, const double* h,
+                   double eta, double* rhores2, hipStream_t st, bool device_api) {
+  const char* who = op == PD_SQUARE ? "nabla_u_square" : op == PD_EPS ? "nabla_u_epsilon" : "update_rhopol";
+  int rc = poisson_check(nord, n, who);
+  if (rc) return rc;
+  if (!h) return fail(LC_ERR_ARG, "%s: null hgrids", who);
+  if (!u || !out || (op != PD_SQUARE && !in1) || (op == PD_RHOPOL && !rhores2))
+    return fail(LC_ERR_ARG, "%s: null array", who);
+  const size_t np = (size_t)n[0] * n[1] * n[2];
+  Arg<R> au, a1, ao;
+  double* r2 = rhores2;
+  if (device_api) {
+    au.dev = const_cast<R*>(u);
+    a1.dev = const_cast<R*>(in1);
+    ao.dev = out;
+  } else {
+    rc = stage_in(u, np, 1, true, st, &au);
+    if (!rc && op != PD_SQUARE) rc = stage_in(in1, (op == PD_EPS ? 1 : 3) * np, 4, true, st, &a1);
+    if (!rc) rc = stage_in<R>(out, (op == PD_EPS ? 3 : 1) * np, 2, op == PD_RHOPOL, st, &ao);
+    if (!rc && op == PD_RHOPOL && !is_device_ptr(rhores2)) {
+      void* d;
+      rc = pool_get(3, sizeof(double), &d, st);
+      r2 = static_cast<double*>(d);
+    }
+    if (rc) return rc;
+  }
+  hipError_t e;
+  if (op == PD_SQUARE) {
+    e = lc::nabla_u_square<R>(geocode, n, au.dev, ao.dev, nord, h, st);
+  } else if (op == PD_EPS) {
+    e = lc::nabla_u_epsilon<R>(geocode, n, au.dev, ao.dev, a1.dev, nord, h, st);
+  } else {
+    const long nparts = lc::update_rhopol_partials<R>(geocode, n, nord);
+    if (nparts < 0) return fail(LC_ERR_ARG, "%s: nord %d not in {2, 4, 6, 8, 16}", who, nord);
+    void* part;
+    rc = pool_get(0, (size_t)std::max(nparts, 1L) * sizeof(double), &part, st);
+    if (rc) return rc;
+    e = lc::update_rhopol<R>(geocode, n, au.dev, nord, h, (R)eta, a1.dev, ao.dev, static_cast<double*>(part),
+                             nparts, r2, st);
+  }
+  if (e != hipSuccess) return hip_fail(e, who);
+  if (!device_api) {
+    if (ao.host_in) HIPCHK(hipMemcpyAsync(out, ao.dev, ao.count * sizeof(R), hipMemcpyDeviceToHost, st));
+    if (r2 != rhores2) HIPCHK(hipMemcpyAsync(rhores2, r2, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return LC_OK;
+}
+
 }  // namespace
 
 namespace lc {
@@ -1508,6 +1564,37 @@ int lc_nabla_u_and_square(int prec, int geocode, const int* n, const void* u, vo
   return fail(LC_ERR_ARG, "lc_nabla_u_and_square: prec %d not in {4, 8}", prec);
 }
 
+int lc_nabla_u_square(int prec, int geocode, const int* n, const void* u, void* ddu, int nord, const double* h,
+                      void* stream) {
+  if (prec == 8)
+    return run_dielectric<double>(PD_SQUARE, geocode, n, (const double*)u, nullptr, (double*)ddu, nord, h, 0.0,
+                                  nullptr, hs(stream), true);
+  if (prec == 4)
+    return run_dielectric<float>(PD_SQUARE, geocode, n, (const float*)u, nullptr, (float*)ddu, nord, h, 0.0,
+                                 nullptr, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_nabla_u_square: prec %d not in {4, 8}", prec);
+}
+int lc_nabla_u_epsilon(int prec, int geocode, const int* n, const void* u, void* du, int nord, const double* h,
+                       const void* eps, void* stream) {
+  if (prec == 8)
+    return run_dielectric<double>(PD_EPS, geocode, n, (const double*)u, (const double*)eps, (double*)du, nord, h,
+                                  0.0, nullptr, hs(stream), true);
+  if (prec == 4)
+    return run_dielectric<float>(PD_EPS, geocode, n, (const float*)u, (const float*)eps, (float*)du, nord, h, 0.0,
+                                 nullptr, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_nabla_u_epsilon: prec %d not in {4, 8}", prec);
+}
+int lc_update_rhopol(int prec, int geocode, const int* n, const void* u, int nord, const double* h, double eta,
+                     const void* dlogeps, void* rhopol, double* rhores2_dev, void* stream) {
+  if (prec == 8)
+    return run_dielectric<double>(PD_RHOPOL, geocode, n, (const double*)u, (const double*)dlogeps,
+                                  (double*)rhopol, nord, h, eta, rhores2_dev, hs(stream), true);
+  if (prec == 4)
+    return run_dielectric<float>(PD_RHOPOL, geocode, n, (const float*)u, (const float*)dlogeps, (float*)rhopol,
+                                 nord, h, eta, rhores2_dev, hs(stream), true);
+  return fail(LC_ERR_ARG, "lc_update_rhopol: prec %d not in {4, 8}", prec);
+}
+
 // the procedure names the BOAST Poisson generator emits (C flavour, FP64,
 // host or device arrays): Poisson_broker (Poisson2.rb:1951-2043), div_u_i
 // (:2067-2195), nabla_u (:2553-2620), nabla_u_and_square (:2197-2276)
@@ -1527,6 +1614,22 @@ void nabla_u_and_square(int geocode, int n01, int n02, int n03, const double* u,
                         const double* hgrids) {
   const int n[3] = {n01, n02, n03};
   run_poisson<double>(PO_NABLA2, nord, -1, 0, geocode, n, u, du, du2, 1.0, 0.0, hgrids, nullptr, false);
+}
+// Poisson2.rb:2278-2359, :2362-2443, :2446-2550
+void nabla_u_square(int geocode, int n01, int n02, int n03, const double* u, double* ddu, int nord,
+                    const double* hgrids) {
+  const int n[3] = {n01, n02, n03};
+  run_dielectric<double>(PD_SQUARE, geocode, n, u, nullptr, ddu, nord, hgrids, 0.0, nullptr, nullptr, false);
+}
+void nabla_u_epsilon(int geocode, int n01, int n02, int n03, const double* u, double* du, int nord,
+                     const double* hgrids, const double* eps) {
+  const int n[3] = {n01, n02, n03};
+  run_dielectric<double>(PD_EPS, geocode, n, u, eps, du, nord, hgrids, 0.0, nullptr, nullptr, false);
+}
+void update_rhopol(int geocode, int n01, int n02, int n03, const double* u, int nord, const double* hgrids,
+                   double eta, const double* dlogeps, double* rhopol, double* rhores2) {
+  const int n[3] = {n01, n02, n03};
+  run_dielectric<double>(PD_RHOPOL, geocode, n, u, dlogeps, rhopol, nord, hgrids, eta, rhores2, nullptr, false);
 }
 
 long lc_local_hamiltonian_work(const int* n) {

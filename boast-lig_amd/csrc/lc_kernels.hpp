@@ -80,6 +80,19 @@ template <typename R>
 hipError_t div_u_i(int geocode, const int* n, const R* u, R* du, R* cc, int nord, const double* h, hipStream_t st);
 template <typename R>
 hipError_t nabla_u(int geocode, const int* n, const R* u, R* du, R* du2, int nord, const double* h, hipStream_t st);
+template <typename R>
+hipError_t nabla_u_square(int geocode, const int* n, const R* u, R* ddu, int nord, const double* h, hipStream_t st);
+template <typename R>
+hipError_t nabla_u_epsilon(int geocode, const int* n, const R* u, R* du, const R* eps, int nord, const double* h,
+                           hipStream_t st);
+// update_rhopol writes one FP64 partial sum of res^2 per block to part
+// (update_rhopol_partials of them, on the current device) and their sum, in
+// a fixed order, to *rhores2 (device)
+template <typename R>
+long update_rhopol_partials(int geocode, const int* n, int nord);
+template <typename R>
+hipError_t update_rhopol(int geocode, const int* n, const R* u, int nord, const double* h, R eta, const R* dlogeps,
+                         R* rhopol, double* part, long nparts, double* rhores2, hipStream_t st);
 
 template <typename R>
 hipError_t sum3d(int id, const Box3& b, int narr, const R* x, R* y, const double* a, R a_x, R a_y,

@@ -18,6 +18,10 @@
 //   div_u_i:             du = sum_k D_k u_k / h_k, optional cc
 //   nabla_u:             du_k = D_k u / h_k
 //   nabla_u_and_square:  the same and du2 = |du|^2
+//   nabla_u_square:      du2 only
+//   nabla_u_epsilon:     du_k = eps D_k u / h_k
+//   update_rhopol:       res = eta ((dlogeps . du) / (4 pi) - rhopol),
+//                        rhopol += res, rhores2 = sum res^2
 // The derivative sums run in increasing tap order, as the reference loops;
 // the 1 / h_k factors are products with the rounded reciprocals (within an
 // ulp of the quotients: a division is a dozen FP64 instructions).
@@ -198,22 +202,94 @@ __global__ __launch_bounds__(kPThreads) void div_u_i_kernel(PGeom g, int segs, c
   }
 }
 
-template <typename R, int NORD>
-__global__ __launch_bounds__(kPThreads) void nabla_u_kernel(PGeom g, int segs, const R* __restrict__ u,
-                                                            R* __restrict__ du, R* __restrict__ du2, R rh0, R rh1, R rh2) {
+// What the gradient kernels do with the scaled derivatives (dx, dy, dz) of
+// point p: MODE 2 (nabla_u) du_k = d_k, and du2 = (dx^2 + dy^2) + dz^2 when
+// du2 is given; 3 (nabla_u_square) du2 only; 4 (nabla_u_epsilon) du_k =
+// d_k eps; 5 (update_rhopol) res = ((e1 dx + e2 dy) + e3 dz) / (4 pi) with
+// (e1, e2, e3) = dlogeps(:, p), res = (res - rhopol) eta, rhopol = res +
+// rhopol, and res^2 returned for the block's partial sum.  The products and
+// sums follow the reference's order (Poisson2.rb:2278-2550).
+enum { GE_GRAD = 2, GE_SQUARE = 3, GE_EPS = 4, GE_RHOPOL = 5 };
+template <typename R>
+struct GradEpi {
+  R* du;
+  R* du2;
+  const R* eps;
+  const R* dlogeps;  // (3, n0, n1, n2), the component fastest
+  R* rhopol;
+  double* part;      // update_rhopol: one partial sum of res^2 per block
+  R eta;
+};
+
+template <typename R, int MODE>
+__device__ __forceinline__ double grad_epilogue(const GradEpi<R>& e, long np, long p, R dx, R dy, R dz) {
+  if constexpr (MODE == GE_GRAD) {
+    e.du[p] = dx;
+    e.du[np + p] = dy;
+    e.du[2 * np + p] = dz;
+    if (e.du2) e.du2[p] = (dx * dx + dy * dy) + dz * dz;  // uniform branch
+  } else if constexpr (MODE == GE_SQUARE) {
+    e.du2[p] = (dx * dx + dy * dy) + dz * dz;
+  } else if constexpr (MODE == GE_EPS) {
+    const R ep = e.eps[p];
+    e.du[p] = dx * ep;
+    e.du[np + p] = dy * ep;
+    e.du[2 * np + p] = dz * ep;
+  } else {
+    const R* d = e.dlogeps + 3 * p;
+    const R oneo4pi = (R)0.07957747154594766788444188168625718101722982287022822437383367203;  // 1 / (4 pi)
+    R res = (d[0] * dx + d[1] * dy) + d[2] * dz;
+    res = res * oneo4pi;
+    const R rho = e.rhopol[p];
+    res = (res - rho) * e.eta;
+    e.rhopol[p] = res + rho;
+    return (double)(res * res);
+  }
+  return 0.0;
+}
+
+// sum of v over a block of NT threads in a fixed order (shuffle tree in each
+// wave, then the waves in order); the result in thread 0
+template <int NT>
+__device__ __forceinline__ double pblock_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
+  return s;
+}
+
+template <typename R, int NORD, int MODE>
+__global__ __launch_bounds__(kPThreads) void grad_point_kernel(PGeom g, int segs, const R* __restrict__ u,
+                                                               GradEpi<R> ep, R rh0, R rh1, R rh2) {
   long p0;
   int count;
-  if (!pblock(g, segs, p0, count) || (int)threadIdx.x >= count) return;
-  const long p = p0 + threadIdx.x;
-  int i0, i1, i2;
-  pcoords(g, p, i0, i1, i2);
-  const R dx = dk<R, NORD>(g, u, p, 0, i0, i1, i2) * rh0;
-  const R dy = dk<R, NORD>(g, u, p, 1, i0, i1, i2) * rh1;
-  const R dz = dk<R, NORD>(g, u, p, 2, i0, i1, i2) * rh2;
-  du[p] = dx;
-  du[g.np + p] = dy;
-  du[2 * g.np + p] = dz;
-  if (du2) du2[p] = (dx * dx + dy * dy) + dz * dz;  // uniform branch
+  const bool blk = pblock(g, segs, p0, count);
+  if constexpr (MODE != GE_RHOPOL) {
+    if (!blk || (int)threadIdx.x >= count) return;
+  } else if (!blk) {  // uniform: the block's partial is 0
+    if (threadIdx.x == 0) ep.part[blockIdx.x] = 0.0;
+    return;
+  }
+  double sq = 0.0;
+  if ((int)threadIdx.x < count) {
+    const long p = p0 + threadIdx.x;
+    int i0, i1, i2;
+    pcoords(g, p, i0, i1, i2);
+    const R dx = dk<R, NORD>(g, u, p, 0, i0, i1, i2) * rh0;
+    const R dy = dk<R, NORD>(g, u, p, 1, i0, i1, i2) * rh1;
+    const R dz = dk<R, NORD>(g, u, p, 2, i0, i1, i2) * rh2;
+    sq = grad_epilogue<R, MODE>(ep, g.np, p, dx, dy, dz);
+  }
+  if constexpr (MODE == GE_RHOPOL) {
+    __shared__ double red[kPThreads / 64];
+    const double s = pblock_sum<kPThreads>(sq, red);
+    if (threadIdx.x == 0) ep.part[blockIdx.x] = s;
+  }
 }
 
 // ------------------------------------------------------------ z-marching form
@@ -235,7 +311,7 @@ __global__ __launch_bounds__(kPThreads) void nabla_u_kernel(PGeom g, int segs, c
 // pderiv's, so both forms give identical bits.
 constexpr int kZTX = 32, kZTY = 8, kZNT = kZTX * kZTY;
 
-// MODE 0: div_u_i, 1: div_u_i with cc, 2: nabla_u (du2 when given).
+// MODE 0: div_u_i, 1: div_u_i with cc, 2..5: the gradient family (GE_*).
 // Derivatives taken of input k (X, Y in LDS; Z in the register window).
 template <int MODE, int K>
 struct PNeed {
@@ -305,10 +381,10 @@ constexpr int pz_waves() { return MODE == 1 && sizeof(R) == 8 ? 2 : 4; }
 template <typename R, int NORD, int MODE>
 __global__ __launch_bounds__(kZNT) __attribute__((amdgpu_waves_per_eu(pz_waves<R, MODE>()))) void poisson_zm_kernel(
     PGeom g, int tiles_x, int tiles, int zlo, int zhi, int zchunk, const R* __restrict__ u, R* __restrict__ o0,
-    R* __restrict__ o1, R rh0, R rh1, R rh2) {
+    R* __restrict__ o1, GradEpi<R> ep, R rh0, R rh1, R rh2) {
   using G = PzGeo<R, NORD>;
   constexpr int M = G::M, W = G::W, TWP = G::TWP, TSZ = G::TSZ;
-  constexpr int NI = MODE == 2 ? 1 : 3;
+  constexpr int NI = MODE >= GE_GRAD ? 1 : 3;
   __shared__ __align__(16) R lds[NI * TSZ];
   __shared__ R rows[W * W];  // every coefficient row (border lanes index it)
   for (int k = threadIdx.x; k < W * W; k += kZNT) rows[k] = prow<R, NORD>(k);
@@ -398,6 +474,7 @@ __global__ __launch_bounds__(kZNT) __attribute__((amdgpu_waves_per_eu(pz_waves<R
   };
   auto ctr = [&](const R* T) { return T[lr * TWP + lc]; };
 
+  double sq = 0.0;  // update_rhopol: this lane's sum of res^2, z in order
   auto step = [&](int z, int b) {  // b = (z - zb) & 1, static after unrolling
     win[W - 1] = nxt[b];
     t0.store(b, lds);
@@ -413,12 +490,9 @@ __global__ __launch_bounds__(kZNT) __attribute__((amdgpu_waves_per_eu(pz_waves<R
       R dzv = R(0);
 #pragma unroll
       for (int j = 0; j < W; ++j) dzv = fmar(prow<R, NORD>(M * W + j), win[j], dzv);
-      if constexpr (MODE == 2) {
+      if constexpr (MODE >= GE_GRAD) {
         const R dx = sx(lds) * rh0, dy = sy(lds) * rh1, dz2 = dzv * rh2;
-        o0[p] = dx;
-        o0[g.np + p] = dy;
-        o0[2 * g.np + p] = dz2;
-        if (o1) o1[p] = (dx * dx + dy * dy) + dz2 * dz2;
+        sq += grad_epilogue<R, MODE>(ep, g.np, p, dx, dy, dz2);
       } else {
         const R dx = sx(lds) * rh0;
         const R dy = sy(lds + TSZ) * rh1;
@@ -443,6 +517,11 @@ __global__ __launch_bounds__(kZNT) __attribute__((amdgpu_waves_per_eu(pz_waves<R
     if (z + 1 < ze) step(z + 1, 1);
     if (z + 2 < ze) step(z + 2, 0);
     if (z + 3 < ze) step(z + 3, 1);
+  }
+  if constexpr (MODE == GE_RHOPOL) {
+    __shared__ double red[kZNT / 64];
+    const double s = pblock_sum<kZNT>(sq, red);
+    if (threadIdx.x == 0) ep.part[blockIdx.x] = s;
   }
 }
 
@@ -540,7 +619,7 @@ static hipError_t pz_run(const PGeom& g, const R* u, R* o0, R* o1, R r0, R r1, R
   *done = pz_launch(g, NO, pz_resident<poisson_zm_kernel<R, NO, MODE>>(), L);
   if (!*done) return hipSuccess;
   poisson_zm_kernel<R, NO, MODE><<<L.blocks, kZNT, 0, st>>>(g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, o0,
-                                                              o1, r0, r1, r2);
+                                                              o1, GradEpi<R>{}, r0, r1, r2);
   return hipGetLastError();
 }
 
@@ -579,22 +658,98 @@ hipError_t div_u_i(int geocode, const int* n, const R* u, R* du, R* cc, int nord
   return e;
 }
 
+// The launches of one gradient-family operator: the z march over the planes
+// off a non-periodic z border and the one-point kernels on the border planes,
+// or the one-point kernels on every plane.  Returns their block count, which
+// is the number of partial sums update_rhopol writes (ep.part[0 .. count));
+// dry: count only, nothing launched.
+template <typename R, int NO, int MODE>
+static long grad_launch(const PGeom& g, const R* u, const GradEpi<R>& ep, R r0, R r1, R r2, hipStream_t st,
+                        bool dry) {
+  long off = 0;
+  auto point = [&](const PGeom& q) {
+    const unsigned nb = pgrid(q);
+    if (!dry) {
+      GradEpi<R> e = ep;
+      if (e.part) e.part += off;
+      grad_point_kernel<R, NO, MODE><<<nb, kPThreads, 0, st>>>(q, psegs(q), u, e, r0, r1, r2);
+    }
+    off += nb;
+  };
+  PzLaunch L;
+  if (!pz_launch(g, NO, pz_resident<poisson_zm_kernel<R, NO, MODE>>(), L)) {
+    point(g);
+    return off;
+  }
+  if (!dry)
+    poisson_zm_kernel<R, NO, MODE><<<L.blocks, kZNT, 0, st>>>(g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u,
+                                                                nullptr, nullptr, ep, r0, r1, r2);
+  off += L.blocks;
+  pz_border_planes(g, NO, point);
+  return off;
+}
+
+template <typename R, int MODE>
+static hipError_t grad_op(int geocode, const int* n, const R* u, const GradEpi<R>& ep, int nord, const double* h,
+                          hipStream_t st, long* blocks, bool dry) {
+  const PGeom g = pgeom(n, geocode, 0, -1);
+  *blocks = 0;
+  if (g.np == 0) return hipSuccess;
+  const R r0 = h ? (R)(1.0 / h[0]) : R(1), r1 = h ? (R)(1.0 / h[1]) : R(1), r2 = h ? (R)(1.0 / h[2]) : R(1);
+  LC_POISSON_DISPATCH(nord, (*blocks = grad_launch<R, NO, MODE>(g, u, ep, r0, r1, r2, st, dry)))
+  return dry ? hipSuccess : hipGetLastError();
+}
+
 template <typename R>
 hipError_t nabla_u(int geocode, const int* n, const R* u, R* du, R* du2, int nord, const double* h, hipStream_t st) {
-  const PGeom g = pgeom(n, geocode, 0, -1);
-  if (g.np == 0) return hipSuccess;
-  const R r0 = (R)(1.0 / h[0]), r1 = (R)(1.0 / h[1]), r2 = (R)(1.0 / h[2]);
-  auto point = [&](const PGeom& q) -> hipError_t {
-    LC_POISSON_DISPATCH(nord, (nabla_u_kernel<R, NO><<<pgrid(q), kPThreads, 0, st>>>(q, psegs(q), u, du, du2, r0, r1, r2)))
-    return hipGetLastError();
-  };
-  bool done = false;
-  hipError_t e = hipSuccess;
-  LC_POISSON_DISPATCH(nord, (e = pz_run<R, NO, 2>(g, u, du, du2, r0, r1, r2, st, &done)))
+  GradEpi<R> ep{};
+  ep.du = du;
+  ep.du2 = du2;
+  long nb;
+  return grad_op<R, GE_GRAD>(geocode, n, u, ep, nord, h, st, &nb, false);
+}
+
+template <typename R>
+hipError_t nabla_u_square(int geocode, const int* n, const R* u, R* ddu, int nord, const double* h, hipStream_t st) {
+  GradEpi<R> ep{};
+  ep.du2 = ddu;
+  long nb;
+  return grad_op<R, GE_SQUARE>(geocode, n, u, ep, nord, h, st, &nb, false);
+}
+
+template <typename R>
+hipError_t nabla_u_epsilon(int geocode, const int* n, const R* u, R* du, const R* eps, int nord, const double* h,
+                           hipStream_t st) {
+  GradEpi<R> ep{};
+  ep.du = du;
+  ep.eps = eps;
+  long nb;
+  return grad_op<R, GE_EPS>(geocode, n, u, ep, nord, h, st, &nb, false);
+}
+
+template <typename R>
+long update_rhopol_partials(int geocode, const int* n, int nord) {
+  if (!poisson_order_ok(nord)) return -1;
+  long nb = 0;
+  (void)grad_op<R, GE_RHOPOL>(geocode, n, nullptr, GradEpi<R>{}, nord, nullptr, nullptr, &nb, true);
+  return nb;
+}
+
+template <typename R>
+hipError_t update_rhopol(int geocode, const int* n, const R* u, int nord, const double* h, R eta, const R* dlogeps,
+                         R* rhopol, double* part, long nparts, double* rhores2, hipStream_t st) {
+  GradEpi<R> ep{};
+  ep.dlogeps = dlogeps;
+  ep.rhopol = rhopol;
+  ep.part = part;
+  ep.eta = eta;
+  long nb = 0;
+  hipError_t e = grad_op<R, GE_RHOPOL>(geocode, n, u, ep, nord, h, st, &nb, true);
   if (e != hipSuccess) return e;
-  if (!done) return point(g);
-  pz_border_planes(g, nord, [&](const PGeom& q) { if (e == hipSuccess) e = point(q); });
-  return e;
+  if (nb > nparts) return hipErrorInvalidValue;  // the caller sized part by update_rhopol_partials
+  e = grad_op<R, GE_RHOPOL>(geocode, n, u, ep, nord, h, st, &nb, false);
+  if (e != hipSuccess) return e;
+  return sum_partial_sets(part, nb, rhores2, 1, 0, st);
 }
 
 #undef LC_POISSON_DISPATCH
@@ -608,5 +763,17 @@ template hipError_t div_u_i<float>(int, const int*, const float*, float*, float*
 template hipError_t nabla_u<double>(int, const int*, const double*, double*, double*, int, const double*,
                                     hipStream_t);
 template hipError_t nabla_u<float>(int, const int*, const float*, float*, float*, int, const double*, hipStream_t);
+template hipError_t nabla_u_square<double>(int, const int*, const double*, double*, int, const double*, hipStream_t);
+template hipError_t nabla_u_square<float>(int, const int*, const float*, float*, int, const double*, hipStream_t);
+template hipError_t nabla_u_epsilon<double>(int, const int*, const double*, double*, const double*, int,
+                                            const double*, hipStream_t);
+template hipError_t nabla_u_epsilon<float>(int, const int*, const float*, float*, const float*, int, const double*,
+                                           hipStream_t);
+template long update_rhopol_partials<double>(int, const int*, int);
+template long update_rhopol_partials<float>(int, const int*, int);
+template hipError_t update_rhopol<double>(int, const int*, const double*, int, const double*, double, const double*,
+                                          double*, double*, long, double*, hipStream_t);
+template hipError_t update_rhopol<float>(int, const int*, const float*, int, const double*, float, const float*,
+                                         float*, double*, long, double*, hipStream_t);
 
 }  // namespace lc

@@ -484,6 +484,44 @@ def nabla_u_and_square(geocode, n, u, du, du2, nord, h, stream=None):
                                     _ptr(du2), nord, hv.ctypes.data_as(_DP), _stream(stream)))
 
 
+def nabla_u_square(geocode, n, u, ddu, nord, h, stream=None):
+    """lc_nabla_u_square (Poisson2.rb:2278-2359): ddu = |grad u|^2."""
+    hv = _h(h)
+    pr = _prec(u)
+    _dev(u, 'u', pr, _count(n))
+    _dev(ddu, 'ddu', pr, _count(n))
+    _resident(u=u, ddu=ddu)
+    _rc(lib().lc_nabla_u_square(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(ddu), nord,
+                                hv.ctypes.data_as(_DP), _stream(stream)))
+
+
+def nabla_u_epsilon(geocode, n, u, du, nord, h, eps, stream=None):
+    """lc_nabla_u_epsilon (Poisson2.rb:2362-2443): du_k = eps D_k u / h_k."""
+    hv = _h(h)
+    pr = _prec(u)
+    _dev(u, 'u', pr, _count(n))
+    _dev(du, 'du', pr, 3 * _count(n))
+    _dev(eps, 'eps', pr, _count(n))
+    _resident(u=u, du=du, eps=eps)
+    _rc(lib().lc_nabla_u_epsilon(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), _ptr(du), nord,
+                                 hv.ctypes.data_as(_DP), _ptr(eps), _stream(stream)))
+
+
+def update_rhopol(geocode, n, u, nord, h, eta, dlogeps, rhopol, rhores2, stream=None):
+    """lc_update_rhopol (Poisson2.rb:2446-2550): rhopol updated in place;
+    rhores2 a one-element FP64 device tensor that receives the sum of res^2."""
+    hv = _h(h)
+    pr = _prec(u)
+    _dev(u, 'u', pr, _count(n))
+    _dev(dlogeps, 'dlogeps', pr, 3 * _count(n))
+    _dev(rhopol, 'rhopol', pr, _count(n))
+    _dev(rhores2, 'rhores2', 'dot', 1)
+    _resident(u=u, dlogeps=dlogeps, rhopol=rhopol, rhores2=rhores2)
+    _rc(lib().lc_update_rhopol(_prec(u), geocode, _ints(n).ctypes.data_as(_IP), _ptr(u), nord,
+                               hv.ctypes.data_as(_DP), _D(eta), _ptr(dlogeps), _ptr(rhopol), _ptr(rhores2),
+                               _stream(stream)))
+
+
 FIL_MD, FIL_IMD, FIL_D2, FIL_D1 = 0, 1, 2, 3
 WAV_SYM8, WAV_SYMICOMB8 = 0, 1
 

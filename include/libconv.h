@@ -362,9 +362,24 @@ int lc_nabla_u(int prec, int geocode, const int *n, const void *u, void *du, int
 /* the gradient and du2 = |grad u|^2 */
 int lc_nabla_u_and_square(int prec, int geocode, const int *n, const void *u, void *du, void *du2, int nord,
                           const double *h, void *stream);
+/* ddu = |grad u|^2 alone: (d_x^2 + d_y^2) + d_z^2                         */
+int lc_nabla_u_square(int prec, int geocode, const int *n, const void *u, void *ddu, int nord,
+                      const double *h, void *stream);
+/* du(:, :, :, k) = eps D_k u / h_k                                           */
+int lc_nabla_u_epsilon(int prec, int geocode, const int *n, const void *u, void *du, int nord,
+                       const double *h, const void *eps, void *stream);
+/* per point res = ((e_1 d_x u + e_2 d_y u) + e_3 d_z u) / (4 pi), with
+ * (e_1, e_2, e_3) = dlogeps(:, i0, i1, i2) ((3, n0, n1, n2), the component
+ * fastest); res = eta (res - rhopol); rhopol = res + rhopol (in place);
+ * *rhores2_dev (device FP64) = the sum of res^2 over the grid, summed in FP64
+ * in a fixed order (the same bits on every call)                           */
+int lc_update_rhopol(int prec, int geocode, const int *n, const void *u, int nord, const double *h,
+                     double eta, const void *dlogeps, void *rhopol, double *rhores2_dev, void *stream);
 /* The generated procedure names (C flavour, FP64, host or device arrays,
  * synchronous): Poisson2.rb:1951-2043 (Poisson_broker), :2067-2195 (div_u_i),
- * :2553-2620 (nabla_u), :2197-2276 (nabla_u_and_square).                    */
+ * :2553-2620 (nabla_u), :2197-2276 (nabla_u_and_square), :2278-2359
+ * (nabla_u_square), :2362-2443 (nabla_u_epsilon), :2446-2550 (update_rhopol;
+ * rhores2 host or device).                                                  */
 void Poisson_broker(int nord, int idim, const int *nn, int bc, const double *u, double *du, double a);
 void div_u_i(int geocode, int n01, int n02, int n03, const double *u, double *du, int nord,
              const double *hgrids, double *cc);
@@ -372,6 +387,12 @@ void nabla_u(int geocode, int n01, int n02, int n03, const double *u, double *du
              const double *hgrids);
 void nabla_u_and_square(int geocode, int n01, int n02, int n03, const double *u, double *du, double *du2,
                         int nord, const double *hgrids);
+void nabla_u_square(int geocode, int n01, int n02, int n03, const double *u, double *ddu, int nord,
+                    const double *hgrids);
+void nabla_u_epsilon(int geocode, int n01, int n02, int n03, const double *u, double *du, int nord,
+                     const double *hgrids, const double *eps);
+void update_rhopol(int geocode, int n01, int n02, int n03, const double *u, int nord, const double *hgrids,
+                   double eta, const double *dlogeps, double *rhopol, double *rhores2);
 
 /* extent of the input (side 0) or output (side 1) along a dim for an
  * operator: op 0..3 = fil ids, 10 + wav = analysis, 20 + wav = synthesis */

@@ -33,7 +33,9 @@ def test_header_declares_the_drop_in_names():
     names = declared()
     for must in ('d_s0s0_1d_sym8_md', 'd_s0s0_1d_sym8_d2_dotin', 'd_s0s1_1d_sym8', 'd_s1s0_1d_sym8',
                  'd_sym8_md_ld_narr', 'd_sym8_d2', 'd_dwt_sym8', 'd_idwt_sym8', 's0s0_1d_sym8_md_d_',
-                 's0s0_1ds_dot_sym8_d2_d_', 'lc_conv_1d', 'lc_last_error'):
+                 's0s0_1ds_dot_sym8_d2_d_', 'lc_conv_1d', 'lc_last_error', 'nabla_u', 'div_u_i',
+                 'nabla_u_and_square', 'nabla_u_square', 'nabla_u_epsilon', 'update_rhopol',
+                 'lc_nabla_u_square', 'lc_nabla_u_epsilon', 'lc_update_rhopol'):
         assert must in names, must
 
 
@@ -155,3 +157,56 @@ def test_noalloc_entry_point_refuses_oversized_partials_before_the_device(lib):
     assert rc == 2  # LC_ERR_UNSUPPORTED
     assert L.lc_last_error() == 2
     assert (buf == 0).all()
+
+
+@pytest.mark.parametrize('case', ['nord', 'small_n', 'null_h', 'null_rhores2', 'prec'])
+def test_poisson_dielectric_entry_points_validate_before_the_device(lib, case):
+    """lc_nabla_u_square, lc_nabla_u_epsilon and lc_update_rhopol refuse a
+    bad order, a grid no larger than the stencil, a missing hgrids or output
+    and a bad precision before any device call (no GPU here)."""
+    import libconv
+    L = libconv.lib()
+    n = (ctypes.c_int * 3)(9, 9, 9)
+    h = (ctypes.c_double * 3)(0.1, 0.1, 0.1)
+    nord, prec, hp = 4, 8, h
+    dummy = ctypes.c_void_p(16)  # never dereferenced: validation fails first
+    r2 = ctypes.cast(dummy, ctypes.POINTER(ctypes.c_double))
+    if case == 'nord':
+        nord = 5
+    elif case == 'small_n':
+        nord = 8  # n = 9 is not > 9
+        n = (ctypes.c_int * 3)(9, 8, 9)
+    elif case == 'null_h':
+        hp = None
+    elif case == 'null_rhores2':
+        r2 = None
+    elif case == 'prec':
+        prec = 2
+    calls = [lambda: L.lc_update_rhopol(prec, 1, n, dummy, nord, hp, ctypes.c_double(0.5), dummy, dummy, r2, None)]
+    if case != 'null_rhores2':
+        calls += [lambda: L.lc_nabla_u_square(prec, 1, n, dummy, dummy, nord, hp, None),
+                  lambda: L.lc_nabla_u_epsilon(prec, 1, n, dummy, dummy, nord, hp, dummy, None)]
+    for call in calls:
+        with pytest.raises(libconv.LibconvError):
+            libconv._rc(call())
+
+
+@pytest.mark.parametrize('case', ['rhores2_dtype', 'dlogeps_short', 'eps_dtype', 'host'])
+def test_poisson_dielectric_wrappers_check_tensors(lib, case):
+    torch = pytest.importorskip('torch')
+    import libconv
+    n = [9, 9, 9]
+    np_ = 729
+    u = torch.zeros(np_, dtype=torch.float64)
+    with pytest.raises(libconv.LibconvError):
+        if case == 'rhores2_dtype':
+            libconv.update_rhopol(1, n, u, 4, [0.1] * 3, 0.5, torch.zeros(3 * np_, dtype=torch.float64),
+                                  torch.zeros(np_, dtype=torch.float64), torch.zeros(1, dtype=torch.float32))
+        elif case == 'dlogeps_short':
+            libconv.update_rhopol(1, n, u, 4, [0.1] * 3, 0.5, torch.zeros(3 * np_ - 1, dtype=torch.float64),
+                                  torch.zeros(np_, dtype=torch.float64), torch.zeros(1, dtype=torch.float64))
+        elif case == 'eps_dtype':
+            libconv.nabla_u_epsilon(1, n, u, torch.zeros(3 * np_, dtype=torch.float64), 4, [0.1] * 3,
+                                    torch.zeros(np_, dtype=torch.float32))
+        else:  # CPU tensors of the right shape: refused as not on the device
+            libconv.nabla_u_square(1, n, u, torch.zeros(np_, dtype=torch.float64), 4, [0.1] * 3)

@@ -3,6 +3,7 @@ FP64 oracle and the reference's own outputs (tests/golden/poisson.npz).
 
 Bar: FP64 within 1e-13 relative max-norm of the oracle; FP32 within 1e-6 of
 the FP64 oracle on the FP32-rounded input (measured on MI355X: <= 3.3e-7)."""
+import ctypes
 import os
 
 import numpy as np
@@ -156,3 +157,161 @@ def test_z_march_equals_point_kernels(lc, gpu, monkeypatch, nord, geocode, dtype
     pt = run()
     for a, b in zip(zm, pt):
         assert torch.equal(a, b)
+
+
+def _dielectric_inputs(rng, n, f32):
+    np_ = int(np.prod(n))
+    u = rng.uniform(-1, 1, np_)
+    eps = rng.uniform(0.5, 2.0, np_)
+    dl = rng.uniform(-1, 1, 3 * np_)
+    rp = rng.uniform(-1, 1, np_)
+    if f32:
+        u, eps, dl, rp = (a.astype(np.float32).astype(np.float64) for a in (u, eps, dl, rp))
+    return u, eps, dl, rp
+
+
+def _run_dielectric(lc, torch, geocode, n, nord, h, eta, u, eps, dl, rp, dtype):
+    """The three operators on the device; returns ddu, du_eps, rhopol (numpy
+    FP64) and rhores2."""
+    np_ = int(np.prod(n))
+    tu = torch.from_numpy(u).to('cuda', dtype)
+    tsq = torch.empty(np_, dtype=dtype, device='cuda')
+    lc.nabla_u_square(geocode, n, tu, tsq, nord, h)
+    tde = torch.empty(3 * np_, dtype=dtype, device='cuda')
+    lc.nabla_u_epsilon(geocode, n, tu, tde, nord, h, torch.from_numpy(eps).to('cuda', dtype))
+    trp = torch.from_numpy(rp).to('cuda', dtype)
+    r2 = torch.full((1,), -1.0, dtype=torch.float64, device='cuda')
+    lc.update_rhopol(geocode, n, tu, nord, h, eta, torch.from_numpy(dl).to('cuda', dtype), trp, r2)
+    torch.cuda.synchronize()
+    return (tsq.cpu().double().numpy(), tde.cpu().double().numpy(), trp.cpu().double().numpy(),
+            float(r2.cpu()[0]))
+
+
+@pytest.mark.parametrize('nord', [2, 4, 6, 8, 16])
+@pytest.mark.parametrize('geocode', [1, 2, 3])
+@pytest.mark.parametrize('dtype_name', ['float64', 'float32'])
+def test_dielectric_operators_match_oracle(lc, gpu, nord, geocode, dtype_name):
+    """nabla_u_square, nabla_u_epsilon and update_rhopol (rhopol in place and
+    rhores2) against the oracle.  rhores2 is summed in another order than the
+    oracle's sequential loop: FP64 within 1e-12 relative, FP32 (squares of
+    FP32 residues, summed in FP64) within 1e-5."""
+    torch = gpu
+    f32 = dtype_name == 'float32'
+    dtype = getattr(torch, dtype_name)
+    rng = np.random.default_rng(4000 + 10 * nord + geocode)
+    n = [nord + 7, nord + 4, nord + 9]
+    np_ = int(np.prod(n))
+    h = [0.15, 0.2, 0.25]
+    eta = 0.7
+    u, eps, dl, rp = _dielectric_inputs(rng, n, f32)
+    sq, de, rpo = np.zeros(np_), np.zeros(3 * np_), rp.copy()
+    O.nabla_u_square(geocode, n, u, sq, nord, h)
+    O.nabla_u_epsilon(geocode, n, u, de, nord, h, eps)
+    r2 = O.update_rhopol(geocode, n, u, nord, h, eta, dl, rpo)
+    gsq, gde, grp, gr2 = _run_dielectric(lc, torch, geocode, n, nord, h, eta, u, eps, dl, rp, dtype)
+    tol = 1e-6 if f32 else TOL64
+    assert rel(gsq, sq) < tol
+    assert rel(gde, de) < tol
+    assert rel(grp, rpo) < tol
+    assert abs(gr2 - r2) <= (1e-5 if f32 else 1e-12) * r2
+
+
+def test_dielectric_against_reference_outputs(lc, gpu):
+    """The library's three operators on the fields of
+    tests/golden/poisson_dielectric.npz against nabla_u_square_ref,
+    nabla_u_epsilon_ref and update_rhopol_ref's outputs for the same fields."""
+    torch = gpu
+    g = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'poisson_dielectric.npz'))
+    h = [float(v) for v in g['h']]
+    eta = float(g['eta'])
+    f = lambda a: np.ascontiguousarray(a.reshape(-1, order='F'))  # noqa: E731
+    for case in g['cases']:
+        c = str(case)
+        nord, geo = int(c.split('_')[0]), int(c.split('_')[1])
+        key = 'big' if nord == 16 else 'small'
+        n = [int(v) for v in g['n_' + key]]
+        gsq, gde, grp, gr2 = _run_dielectric(lc, torch, geo, n, nord, h, eta, f(g['u_' + key]),
+                                             f(g['eps_' + key]), f(g['dlogeps_' + key]), f(g['rhopol_' + key]),
+                                             torch.float64)
+        assert rel(gsq, f(g['square_' + c])) < TOL64, c
+        assert rel(gde, f(g['epsilon_' + c])) < TOL64, c
+        assert rel(grp, f(g['rhopol_' + c])) < TOL64, c
+        assert abs(gr2 - float(g['rhores2_' + c])) <= 1e-12 * abs(gr2), c
+
+
+@pytest.mark.parametrize('nord', [4, 16])
+def test_dielectric_drop_in_names(lc, gpu, nord):
+    """nabla_u_square, nabla_u_epsilon and update_rhopol under the generated
+    procedure names, on host arrays (staged, synchronous; rhores2 on the
+    host) and with device arrays, against the oracle."""
+    torch = gpu
+    rng = np.random.default_rng(4100 + nord)
+    n = [nord + 6, nord + 5, nord + 3]
+    np_ = int(np.prod(n))
+    h = np.array([0.3, 0.2, 0.1])
+    u, eps, dl, rp = _dielectric_inputs(rng, n, False)
+    L, P = lc.lib(), lc._DP
+    sq, de, rpo = np.zeros(np_), np.zeros(3 * np_), rp.copy()
+    O.nabla_u_square(3, n, u, sq, nord, h)
+    O.nabla_u_epsilon(3, n, u, de, nord, h, eps)
+    r2 = O.update_rhopol(3, n, u, nord, h, 0.5, dl, rpo)
+    hsq, hde, hrp, hr2 = np.zeros(np_), np.zeros(3 * np_), rp.copy(), np.zeros(1)
+    L.nabla_u_square(3, n[0], n[1], n[2], u.ctypes.data_as(P), hsq.ctypes.data_as(P), nord, h.ctypes.data_as(P))
+    lc.check()
+    L.nabla_u_epsilon(3, n[0], n[1], n[2], u.ctypes.data_as(P), hde.ctypes.data_as(P), nord, h.ctypes.data_as(P),
+                      eps.ctypes.data_as(P))
+    lc.check()
+    L.update_rhopol(3, n[0], n[1], n[2], u.ctypes.data_as(P), nord, h.ctypes.data_as(P), lc._D(0.5),
+                    dl.ctypes.data_as(P), hrp.ctypes.data_as(P), hr2.ctypes.data_as(P))
+    lc.check()
+    assert rel(hsq, sq) < TOL64 and rel(hde, de) < TOL64 and rel(hrp, rpo) < TOL64
+    assert abs(hr2[0] - r2) <= 1e-12 * r2
+    # device arrays through the same names (rhores2 on the device)
+    tu, tdl = torch.from_numpy(u).cuda(), torch.from_numpy(dl).cuda()
+    trp, tr2 = torch.from_numpy(rp).cuda(), torch.zeros(1, dtype=torch.float64, device='cuda')
+    L.update_rhopol(3, n[0], n[1], n[2], lc._ptr(tu), nord, h.ctypes.data_as(P), lc._D(0.5), lc._ptr(tdl),
+                    lc._ptr(trp), ctypes.cast(lc._ptr(tr2), P))
+    lc.check()
+    torch.cuda.synchronize()
+    assert np.array_equal(trp.cpu().numpy(), hrp) and float(tr2.cpu()[0]) == hr2[0]
+
+
+@pytest.mark.parametrize('nord', [4, 16])
+@pytest.mark.parametrize('geocode', [1, 2])
+@pytest.mark.parametrize('dtype_name', ['float64', 'float32'])
+def test_dielectric_z_march_equals_point_kernels(lc, gpu, monkeypatch, nord, geocode, dtype_name):
+    """The z-marching and one-point forms of the three operators give the same
+    bits (rhores2: the same value within rounding, the partial sums being
+    laid out per block), and rhores2 repeats bit for bit from call to call."""
+    torch = gpu
+    dtype = getattr(torch, dtype_name)
+    rng = np.random.default_rng(4200 + nord + geocode)
+    n = [70, 37, 2 * nord + 45]
+    h = [0.15, 0.2, 0.25]
+    u, eps, dl, rp = _dielectric_inputs(rng, n, dtype_name == 'float32')
+    zm = _run_dielectric(lc, torch, geocode, n, nord, h, 0.3, u, eps, dl, rp, dtype)
+    again = _run_dielectric(lc, torch, geocode, n, nord, h, 0.3, u, eps, dl, rp, dtype)
+    assert again[3] == zm[3]
+    monkeypatch.setenv('LIBCONV_POISSON_POINT', '1')
+    pt = _run_dielectric(lc, torch, geocode, n, nord, h, 0.3, u, eps, dl, rp, dtype)
+    for a, b in zip(zm[:3], pt[:3]):
+        assert np.array_equal(a, b)
+    assert abs(zm[3] - pt[3]) <= 1e-12 * zm[3]
+
+
+def test_dielectric_large_grid(lc, gpu):
+    """A 200 x 180 x 160 grid (5.8 M points, many tiles and z chunks) in FP64
+    against the oracle, nord 16, x and z periodic, y free."""
+    torch = gpu
+    rng = np.random.default_rng(4300)
+    n = [200, 180, 160]
+    h = [0.1, 0.12, 0.14]
+    u, eps, dl, rp = _dielectric_inputs(rng, n, False)
+    np_ = int(np.prod(n))
+    sq, de, rpo = np.zeros(np_), np.zeros(3 * np_), rp.copy()
+    O.nabla_u_square(3, n, u, sq, 16, h)
+    O.nabla_u_epsilon(3, n, u, de, 16, h, eps)
+    r2 = O.update_rhopol(3, n, u, 16, h, 0.5, dl, rpo)
+    gsq, gde, grp, gr2 = _run_dielectric(lc, torch, 3, n, 16, h, 0.5, u, eps, dl, rp, torch.float64)
+    assert rel(gsq, sq) < TOL64 and rel(gde, de) < TOL64 and rel(grp, rpo) < TOL64
+    assert abs(gr2 - r2) <= 1e-12 * r2
