@@ -246,32 +246,49 @@ def run_hamiltonian(torch, lc, dtype, steps, warmup, n=64):
             'fine_points_per_s_G': round(nf / (ms * 1e-3) / 1e9, 3)}
 
 
-def run_poisson(torch, lc, dtype, steps, warmup, n=256, nord=16, geocode=3):
-    """div_u_i with cc on an n^3 vector field, order 16, geocode 3 (x, z
-    periodic, y free): the configuration of BigDFT/Poisson_test2.rb."""
+def run_poisson(torch, lc, dtype, steps, warmup, n=256, nord=16, geocode=3, op='div'):
+    """op 'div': div_u_i with cc on an n^3 vector field, order 16, geocode 3
+    (x, z periodic, y free): the configuration of BigDFT/Poisson_test2.rb.
+    op 'rhopol': update_rhopol on the same grid (u, dlogeps and rhopol in,
+    rhopol out, rhores2 summed on the device)."""
     nn = [n] * 3
     npts = n ** 3
     g = torch.Generator(device='cuda').manual_seed(9)
-    u = torch.rand(3 * npts, dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
-    du = torch.empty(npts, dtype=dtype, device='cuda')
-    cc = torch.empty(npts, dtype=dtype, device='cuda')
     h = [0.1, 0.1, 0.1]
     st = torch.cuda.current_stream()
+    if op == 'div':
+        u = torch.rand(3 * npts, dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+        du = torch.empty(npts, dtype=dtype, device='cuda')
+        cc = torch.empty(npts, dtype=dtype, device='cuda')
+
+        def call():
+            lc.div_u_i(geocode, nn, u, du, nord, h, cc, stream=st)
+        per_point = 5  # u (3 components) in, du and cc out
+    else:
+        u = torch.rand(npts, dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+        dl = torch.rand(3 * npts, dtype=torch.float64, device='cuda', generator=g).mul_(2).sub_(1).to(dtype)
+        rp = torch.zeros(npts, dtype=dtype, device='cuda')
+        r2 = torch.zeros(1, dtype=torch.float64, device='cuda')
+
+        def call():
+            # rhopol moves from step to step; the work of a step does not
+            lc.update_rhopol(geocode, nn, u, nord, h, 0.5, dl, rp, r2, stream=st)
+        per_point = 6  # u, dlogeps (3), rhopol in; rhopol out
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     with quiet_host():
-        settle_gpu(torch, lambda i: lc.div_u_i(geocode, nn, u, du, nord, h, cc, stream=st), st)
+        settle_gpu(torch, lambda i: call(), st)
         for _ in range(warmup):
-            lc.div_u_i(geocode, nn, u, du, nord, h, cc, stream=st)
+            call()
         torch.cuda.synchronize()
         e0.record(st)
         for _ in range(steps):
-            lc.div_u_i(geocode, nn, u, du, nord, h, cc, stream=st)
+            call()
         e1.record(st)
         torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     sz = 4 if dtype == torch.float32 else 8
-    byts = sz * 5 * npts  # u (3 components) in, du and cc out
-    return {'n': nn, 'nord': nord, 'geocode': geocode, 'ms': round(ms, 5),
+    byts = sz * per_point * npts
+    return {'op': op, 'n': nn, 'nord': nord, 'geocode': geocode, 'ms': round(ms, 5),
             'GB_s': round(byts / (ms * 1e-3) / 1e9, 1),
             'frac_hbm': round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), 'algorithmic_bytes': byts,
             'Gpts_s': round(npts / (ms * 1e-3) / 1e9, 3)}
@@ -650,6 +667,7 @@ def main():
         out['hamiltonian']['fp32'] = run_hamiltonian(torch, lc, torch.float32, args.steps, args.warmup)
         out['poisson_div'] = run_poisson(torch, lc, torch.float64, args.steps, args.warmup)
         out['poisson_div']['fp32'] = run_poisson(torch, lc, torch.float32, args.steps, args.warmup)
+        out['poisson_rhopol'] = run_poisson(torch, lc, torch.float64, args.steps, args.warmup, op='rhopol')
         out['batch'] = run_batch(torch, lc, torch.float64, reps=3)
         out['batch']['fp32'] = run_batch(torch, lc, torch.float32, reps=3)
     if not args.no_oracle:
