@@ -210,3 +210,16 @@ def test_poisson_dielectric_wrappers_check_tensors(lib, case):
                                     torch.zeros(np_, dtype=torch.float32))
         else:  # CPU tensors of the right shape: refused as not on the device
             libconv.nabla_u_square(1, n, u, torch.zeros(np_, dtype=torch.float64), 4, [0.1] * 3)
+
+
+def test_static_archive_links_and_runs(tmp_path):
+    """libconv_amd.a (make static) links into a C program with the HIP
+    runtime and serves the calls that need no device."""
+    pkg = os.path.join(ROOT, 'boast-lig_amd')
+    subprocess.run(['make', '-s', '-C', pkg, 'static'], check=True, capture_output=True)
+    exe = str(tmp_path / 'lc_static_check')
+    subprocess.run(['gcc', '-O1', '-I', os.path.join(ROOT, 'include'), os.path.join(ROOT, 'tests', 'c',
+                    'lc_static_check.c'), os.path.join(pkg, 'libconv_amd.a'), '-L/opt/rocm/lib', '-lamdhip64',
+                    '-lstdc++', '-lm', '-Wl,-rpath,/opt/rocm/lib', '-o', exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip().endswith('OK'), out.stdout + out.stderr
