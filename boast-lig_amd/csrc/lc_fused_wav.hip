@@ -25,6 +25,7 @@
 #include <string>
 
 #include "lc_body_wav.hpp"
+#include "lc_body_wavq.hpp"
 
 namespace lc {
 
@@ -50,6 +51,36 @@ __global__ __launch_bounds__(2 * TX * TY) __attribute__((amdgpu_waves_per_eu(WPE
   sepw3d_body<R, Op, TX, TY, QX, BZ, AD>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
 }
 
+// the plane-at-a-time form (lc_body_wavq.hpp): x-waves and y-waves, one
+// barrier per input plane
+template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE>
+__global__ __launch_bounds__((WavqGeo<R, Op, TX, TY, QX>::NT)) __attribute__((amdgpu_waves_per_eu(WPE))) void wavq_fused(
+    FusedArgs<R> p) {
+  __shared__ __align__(16) unsigned char lds[WavqGeo<R, Op, TX, TY, QX>::lds_bytes()];
+  int bx, by, zc, arr;
+  block_coords(p.tiles_x, bx, by, zc, arr);
+  if (p.flags & F_AY)
+    wavq_body<R, Op, TX, TY, QX, PD, true>(p, bx, by, zc, arr, lds);
+  else
+    wavq_body<R, Op, TX, TY, QX, PD, false>(p, bx, by, zc, arr, lds);
+}
+
+template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE>
+__global__ __launch_bounds__((WavqGeo<R, Op, TX, TY, QX>::NT)) __attribute__((amdgpu_waves_per_eu(WPE))) void wavq_batch(
+    const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
+  const int lin = xcd_linear();
+  const int o = batch_item(first, count, lin);
+  const FusedArgs<R> p = tab[o];
+  const int local = lin - first[o];
+  const int tiles = p.tiles_x * p.tiles_y;
+  const int tile = local % tiles, zc = local / tiles;
+  __shared__ __align__(16) unsigned char lds[WavqGeo<R, Op, TX, TY, QX>::lds_bytes()];
+  if (p.flags & F_AY)
+    wavq_body<R, Op, TX, TY, QX, PD, true>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+  else
+    wavq_body<R, Op, TX, TY, QX, PD, false>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+}
+
 // ====================================================================== host
 // Compiled variants: tile TX x TY steps, QX outputs per x-task, BZ steps per
 // window slide, AD prefetch depth (2 TX TY lanes per block), waves per EU,
@@ -59,7 +90,19 @@ __global__ __launch_bounds__(2 * TX * TY) __attribute__((amdgpu_waves_per_eu(WPE
   X(0, 16, 8, 4, 2, 2, 1, 0) X(1, 8, 8, 4, 2, 2, 1, 0) X(2, 32, 4, 4, 2, 2, 1, 0) X(3, 16, 8, 4, 1, 1, 1, 0) \
   X(4, 16, 8, 4, 1, 1, 2, 0) X(5, 16, 8, 4, 1, 1, 1, -2) X(6, 16, 8, 4, 2, 2, 1, -2)
 
-int wav_variant_count() { return 7; }
+// plane-at-a-time variants: tile TX x TY steps, QX outputs per x-task, PD
+// planes of prefetch, waves per EU, z chunks as above
+#define LC_WAVQ_VARIANTS(X)                                                                       \
+  X(7, 16, 8, 4, 2, 1, -2) X(8, 16, 8, 4, 2, 1, -3) X(9, 32, 8, 4, 2, 1, -2) X(10, 16, 16, 4, 2, 1, -2) \
+  X(11, 16, 8, 4, 4, 1, -2)
+
+int wav_variant_count() { return 12; }
+
+// output planes reached through 32-bit buffer records (the plane-at-a-time
+// variants store with buffer stores)
+static bool wavq_fits(const Box3& b, int prec) {
+  return wav_plane_fits(b, prec) && 4L * b.ny[0] * b.ny[1] * prec <= kPlaneMax;
+}
 
 std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
   (void)id;
@@ -75,6 +118,13 @@ std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
              r, op, TX, TY, QX, BZ, AD, WPE);                                                        \
     break;
     LC_WAV_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, PD, WPE, ZC)                                                                       \
+  case I:                                                                                                   \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d>", batch ? "wavq_batch" : "wavq_fused", r, op, TX, \
+             TY, QX, PD, WPE);                                                                              \
+    break;
+    LC_WAVQ_VARIANTS(X)
 #undef X
   }
   return buf;
@@ -121,6 +171,34 @@ static hipError_t launch_sepw3d(const Box3& b, int narr, const R* x, R* y, R a, 
   return hipGetLastError();
 }
 
+// z chunk of a single call: ZC < 0: at most -ZC blocks on any CU, at least
+// one where the grid allows, chunks >= W steps; else ~2 blocks per CU,
+// chunks >= 2 W steps
+template <class Op, int ZC>
+static int wav_zchunk(long tiles, int narr, int nout) {
+  if (ZC < 0) {
+    const long per = tiles * std::max(narr, 1);
+    long chunks = (long)(-ZC) * num_cu() / per;
+    if (chunks * per < num_cu()) chunks = (num_cu() + per - 1) / per;
+    chunks = std::max(1L, std::min<long>(chunks, std::max(1, nout / Op::W)));
+    return (int)((nout + chunks - 1) / chunks);
+  }
+  return pick_zchunk(tiles, nout, std::max(narr, 1), std::max(1, Op::W / 2), num_cu());
+}
+
+template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE, int ZC>
+static hipError_t launch_wavq(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
+  if (!wavq_fits(b, sizeof(R))) return hipErrorInvalidValue;
+  FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, 1, 1>(b, narr, x, y, a, a_y);
+  const long tiles = (long)p.tiles_x * p.tiles_y;
+  p.zchunk = wav_zchunk<Op, ZC>(tiles, narr, p.d2.nout);
+  const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
+  if (narr == 0) return hipSuccess;
+  dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
+  wavq_fused<R, Op, TX, TY, QX, PD, WPE><<<grid, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(p);
+  return hipGetLastError();
+}
+
 template <typename R, class Op>
 static hipError_t sepw3d_variant(int v, const Box3& b, int narr, const R* x, R* y, R a, R a_y,
                                  hipStream_t st) {
@@ -128,6 +206,10 @@ static hipError_t sepw3d_variant(int v, const Box3& b, int narr, const R* x, R* 
 #define X(I, TX, TY, QX, BZ, AD, WPE, ZC) \
   case I: return launch_sepw3d<R, Op, TX, TY, QX, BZ, AD, WPE, ZC>(b, narr, x, y, a, a_y, st);
     LC_WAV_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, PD, WPE, ZC) \
+  case I: return launch_wavq<R, Op, TX, TY, QX, PD, WPE, ZC>(b, narr, x, y, a, a_y, st);
+    LC_WAVQ_VARIANTS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -190,6 +272,13 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
   case I: return wav_stage_build<R, Op, TX, TY, QX, BZ, AD, WPE>(it, count, s);
       LC_WAV_VARIANTS(X)
 #undef X
+#define X(I, TX, TY, QX, PD, WPE, ZC)                                      \
+  case I:                                                                  \
+    for (int o = 0; o < count; ++o)                                        \
+      if (!wavq_fits(it[o].b, sizeof(R))) return hipErrorInvalidValue;     \
+    return wav_stage_build<R, Op, TX, TY, QX, 1, 1, WPE>(it, count, s);
+      LC_WAVQ_VARIANTS(X)
+#undef X
     }
     return hipErrorInvalidValue;
   }
@@ -201,6 +290,13 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
     sepw3d_batch<R, Op, TX, TY, QX, BZ, AD, WPE><<<s->blocks, 2 * TX * TY, 0, st>>>(tab, s->first, s->count); \
     break;
     LC_WAV_VARIANTS(X)
+#undef X
+#define X(I, TX, TY, QX, PD, WPE, ZC)                                                                     \
+  case I:                                                                                                 \
+    wavq_batch<R, Op, TX, TY, QX, PD, WPE><<<s->blocks, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(tab, s->first, \
+                                                                                                s->count); \
+    break;
+    LC_WAVQ_VARIANTS(X)
 #undef X
     default: return hipErrorInvalidValue;
   }

@@ -64,7 +64,7 @@ def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v, bc):
     assert np.abs(dot.cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
 
 
-@pytest.mark.parametrize('v', range(7))
+@pytest.mark.parametrize('v', range(12))
 @pytest.mark.parametrize('inverse', [False, True])
 def test_every_wavelet_variant_matches_oracle(lc, gpu, monkeypatch, v, inverse):
     torch = gpu
@@ -109,6 +109,36 @@ def test_all_separable_variants_give_identical_bits(lc, gpu, monkeypatch):
         torch.cuda.synchronize()
         for v, y in enumerate(outs):
             assert torch.equal(y, outs[0]), (str(dt), v)
+
+
+@pytest.mark.parametrize('inverse', [False, True])
+def test_all_wavelet_variants_give_identical_bits(lc, gpu, monkeypatch, inverse):
+    """The step-at-a-time and plane-at-a-time wavelet kernels sum in the same
+    order: every variant gives the same bits, in both precisions, with and
+    without a and a_y (a shape with partial tiles, several z chunks and every
+    boundary condition)."""
+    torch = gpu
+    rng = np.random.default_rng(1270 + inverse)
+    n = [21, 19, 30]
+    bc = [1, 0, -1]
+    nx = [n[0], n[1], n[2] + 7]
+    ny = [n[0] + 7, n[1], n[2]]
+    if inverse:
+        nx, ny = ny, nx
+        bc = [-1, 0, 1]
+    for dt in (torch.float64, torch.float32):
+        x = torch.from_numpy(rng.uniform(-1, 1, 8 * int(np.prod(nx)))).to('cuda', dt)
+        y0 = torch.from_numpy(rng.uniform(-1, 1, 8 * int(np.prod(ny)))).to('cuda', dt)
+        for a, a_y in ((1.0, 0.0), (0.75, -0.5)):
+            outs = []
+            for v in range(lc.variant_count(lc.FORM_DWT)):
+                monkeypatch.setenv('LIBCONV_WAV3D_VARIANT', str(v))
+                y = y0.clone()
+                lc.wavelet_md(lc.WAV_SYM8, inverse, n, bc, nx, ny, 1, x, y, a=a, a_y=a_y)
+                outs.append(y)
+            torch.cuda.synchronize()
+            for v, y in enumerate(outs):
+                assert torch.equal(y, outs[0]), (str(dt), a, a_y, v)
 
 
 def test_all_variants_give_identical_bits(lc, gpu, monkeypatch):
@@ -166,7 +196,7 @@ def test_batch_tune_keeps_outputs(lc, gpu):
     before = [it['y'].clone() for it in items]
     ms = plan.tune(repeat=2)
     assert len(ms) == 2 and all(np.isfinite(ms))
-    assert plan.stage_variant(0) not in (5, 6, 7) and 0 <= plan.stage_variant(1) < 4
+    assert plan.stage_variant(0) not in (5, 6, 7) and 0 <= plan.stage_variant(1) < lc.variant_count(lc.FORM_DWT)
     for it in items:
         it['y'].fill_(3.0)
     plan.run_graph()
