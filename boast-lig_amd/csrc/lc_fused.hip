@@ -215,7 +215,13 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     return prec == 4 ? 32 : 46;
   }
   v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
-  return v >= 0 ? v : 5;  // one z-step per slide, at most two blocks per CU
+  if (v >= 0) return v;
+  // FP32: the plane-at-a-time kernel (16 x 8 steps; its outputs go through
+  // 32-bit buffer records); FP64: one z-step per slide, at most two blocks
+  // per CU (the plane-at-a-time kernel's 161 VGPRs leave three waves per
+  // SIMD: slower on 96^3 pairs, faster on 64^3; the tuner picks per shape)
+  if (prec == 4 && wav_plane_fits(b, prec) && 4L * b.ny[0] * b.ny[1] * prec <= kPlaneMax) return 7;
+  return 5;
 }
 
 template <typename R>
