@@ -62,7 +62,11 @@ struct WavqGeo {
 template <int PD>
 constexpr bool wavq_pd_ok() { return PD >= 1 && (2 * 8) % PD == 0; }
 
-template <typename R, class Op, int TX, int TY, int QX, int PD, bool ME = true, bool NTS = false>
+// KB > 0: the x- and y-tasks consume their LDS reads as they arrive, with a
+// scheduling fence every KB 16-byte pieces (x) or KB window steps (y), so that
+// few values are in flight: fewer VGPRs, more waves per SIMD.  The FMAs run
+// in the same order either way.
+template <typename R, class Op, int TX, int TY, int QX, int PD, bool ME = true, bool NTS = false, int KB = 0>
 __device__ __forceinline__ void wavq_body(const FusedArgs<R>& p, int bx, int by, int zc, int arr,
                                           unsigned char* lds) {
   using G = WavqGeo<R, Op, TX, TY, QX>;
@@ -170,24 +174,74 @@ __device__ __forceinline__ void wavq_body(const FusedArgs<R>& p, int bx, int by,
       if (xwave) {
         // ---- x-task on plane s: two runs of 16-byte reads along the row
         const R* row = A + (u & 1) * HA + xo;
-        R v[2][RX * V];
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-          for (int m = 0; m < RX; ++m) lds_read16(row + c * TWSP + m * V, &v[c][m * V]);
         R* bd = B + (u & 1) * HB + bo;
+        if constexpr (KB == 0) {
+          R v[2][RX * V];
 #pragma unroll
-        for (int q = 0; q < QX; ++q) {
-          R o0, o1;
-          wav_step<R, Op>([&](int ss, int c) { return v[c][q + ss]; }, o0, o1);
-          bd[q * BHT] = o0;
-          bd[(TX + q) * BHT] = o1;
+          for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int m = 0; m < RX; ++m) lds_read16(row + c * TWSP + m * V, &v[c][m * V]);
+#pragma unroll
+          for (int q = 0; q < QX; ++q) {
+            R o0, o1;
+            wav_step<R, Op>([&](int ss, int c) { return v[c][q + ss]; }, o0, o1);
+            bd[q * BHT] = o0;
+            bd[(TX + q) * BHT] = o1;
+          }
+        } else {
+          // piece m of both runs feeds every output whose window holds it;
+          // each accumulator still takes its terms step by step, value 0
+          // before value 1 (the order of wav_step)
+          using C = WavCoef<Op>;
+          R t0[QX], t1[QX];
+#pragma unroll
+          for (int q = 0; q < QX; ++q) t0[q] = t1[q] = R(0);
+#pragma unroll
+          for (int m = 0; m < RX; ++m) {
+            if (m > 0 && m % KB == 0) __builtin_amdgcn_sched_barrier(0);
+            R v0[V], v1[V];
+            lds_read16(row + m * V, v0);
+            lds_read16(row + TWSP + m * V, v1);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              const int k = m * V + e;
+#pragma unroll
+              for (int q = 0; q < QX; ++q) {
+                const int ss = k - q;
+                if (ss >= 0 && ss < W) {
+                  t0[q] = fmar(v0[e], C::template c<R>(0, 0, ss), t0[q]);
+                  t1[q] = fmar(v0[e], C::template c<R>(1, 0, ss), t1[q]);
+                  t0[q] = fmar(v1[e], C::template c<R>(0, 1, ss), t0[q]);
+                  t1[q] = fmar(v1[e], C::template c<R>(1, 1, ss), t1[q]);
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < QX; ++q) {
+            bd[q * BHT] = t0[q];
+            bd[(TX + q) * BHT] = t1[q];
+          }
         }
       } else {
         // ---- y-task on plane s - 1: two runs along the column, into the ring
         const R* colp = B + ((u + 1) & 1) * HB + yo;
         R y0, y1;
-        wav_step<R, Op>([&](int ss, int c) { return colp[c * TH + ss]; }, y0, y1);
+        if constexpr (KB == 0) {
+          wav_step<R, Op>([&](int ss, int c) { return colp[c * TH + ss]; }, y0, y1);
+        } else {
+          using C = WavCoef<Op>;
+          y0 = y1 = R(0);
+#pragma unroll
+          for (int ss = 0; ss < W; ++ss) {
+            if (ss > 0 && ss % KB == 0) __builtin_amdgcn_sched_barrier(0);
+            const R a = colp[ss], b = colp[TH + ss];
+            y0 = fmar(a, C::template c<R>(0, 0, ss), y0);
+            y1 = fmar(a, C::template c<R>(1, 0, ss), y1);
+            y0 = fmar(b, C::template c<R>(0, 1, ss), y0);
+            y1 = fmar(b, C::template c<R>(1, 1, ss), y1);
+          }
+        }
         ring[0][(u + NP - 1) % NP] = y0;
         ring[1][(u + NP - 1) % NP] = y1;
         // ---- z: after the second value of an input step, the output step

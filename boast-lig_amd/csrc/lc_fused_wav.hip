@@ -53,19 +53,19 @@ __global__ __launch_bounds__(2 * TX * TY) __attribute__((amdgpu_waves_per_eu(WPE
 
 // the plane-at-a-time form (lc_body_wavq.hpp): x-waves and y-waves, one
 // barrier per input plane
-template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE>
+template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE, int KB>
 __global__ __launch_bounds__((WavqGeo<R, Op, TX, TY, QX>::NT)) __attribute__((amdgpu_waves_per_eu(WPE))) void wavq_fused(
     FusedArgs<R> p) {
   __shared__ __align__(16) unsigned char lds[WavqGeo<R, Op, TX, TY, QX>::lds_bytes()];
   int bx, by, zc, arr;
   block_coords(p.tiles_x, bx, by, zc, arr);
   if (p.flags & F_AY)
-    wavq_body<R, Op, TX, TY, QX, PD, true>(p, bx, by, zc, arr, lds);
+    wavq_body<R, Op, TX, TY, QX, PD, true, false, KB>(p, bx, by, zc, arr, lds);
   else
-    wavq_body<R, Op, TX, TY, QX, PD, false>(p, bx, by, zc, arr, lds);
+    wavq_body<R, Op, TX, TY, QX, PD, false, false, KB>(p, bx, by, zc, arr, lds);
 }
 
-template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE>
+template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE, int KB>
 __global__ __launch_bounds__((WavqGeo<R, Op, TX, TY, QX>::NT)) __attribute__((amdgpu_waves_per_eu(WPE))) void wavq_batch(
     const FusedArgs<R>* __restrict__ tab, const int* __restrict__ first, int count) {
   const int lin = xcd_linear();
@@ -76,9 +76,9 @@ __global__ __launch_bounds__((WavqGeo<R, Op, TX, TY, QX>::NT)) __attribute__((am
   const int tile = local % tiles, zc = local / tiles;
   __shared__ __align__(16) unsigned char lds[WavqGeo<R, Op, TX, TY, QX>::lds_bytes()];
   if (p.flags & F_AY)
-    wavq_body<R, Op, TX, TY, QX, PD, true>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+    wavq_body<R, Op, TX, TY, QX, PD, true, false, KB>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
   else
-    wavq_body<R, Op, TX, TY, QX, PD, false>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
+    wavq_body<R, Op, TX, TY, QX, PD, false, false, KB>(p, tile % p.tiles_x, tile / p.tiles_x, zc, 0, lds);
 }
 
 // ====================================================================== host
@@ -91,12 +91,14 @@ __global__ __launch_bounds__((WavqGeo<R, Op, TX, TY, QX>::NT)) __attribute__((am
   X(4, 16, 8, 4, 1, 1, 2, 0) X(5, 16, 8, 4, 1, 1, 1, -2) X(6, 16, 8, 4, 2, 2, 1, -2)
 
 // plane-at-a-time variants: tile TX x TY steps, QX outputs per x-task, PD
-// planes of prefetch, waves per EU, z chunks as above
-#define LC_WAVQ_VARIANTS(X)                                                                       \
-  X(7, 16, 8, 4, 2, 1, -2) X(8, 16, 8, 4, 2, 1, -3) X(9, 32, 8, 4, 2, 1, -2) X(10, 16, 16, 4, 2, 1, -2) \
-  X(11, 16, 8, 4, 4, 1, -2)
+// planes of prefetch, waves per EU, z chunks as above, scheduling fence
+// interval of the filter reads (0: none)
+#define LC_WAVQ_VARIANTS(X)                                                                                  \
+  X(7, 16, 8, 4, 2, 1, -2, 0) X(8, 16, 8, 4, 2, 1, -3, 0) X(9, 32, 8, 4, 2, 1, -2, 0)                           \
+  X(10, 16, 16, 4, 2, 1, -2, 0) X(11, 16, 8, 4, 4, 1, -2, 0) X(12, 16, 8, 4, 2, 4, -2, 2)                        \
+  X(13, 16, 16, 4, 2, 4, -2, 2) X(14, 16, 8, 4, 2, 4, -2, 1) X(15, 16, 8, 4, 2, 1, -2, 2)
 
-int wav_variant_count() { return 12; }
+int wav_variant_count() { return 16; }
 
 // output planes reached through 32-bit buffer records (the plane-at-a-time
 // variants store with buffer stores)
@@ -119,10 +121,10 @@ std::string wav_kernel_name(int prec, int form, int id, int v, bool batch) {
     break;
     LC_WAV_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, PD, WPE, ZC)                                                                       \
+#define X(I, TX, TY, QX, PD, WPE, ZC, KB)                                                                   \
   case I:                                                                                                   \
-    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d>", batch ? "wavq_batch" : "wavq_fused", r, op, TX, \
-             TY, QX, PD, WPE);                                                                              \
+    snprintf(buf, sizeof buf, "%s<%s, lc::%s, %d, %d, %d, %d, %d, %d>", batch ? "wavq_batch" : "wavq_fused", r, op, \
+             TX, TY, QX, PD, WPE, KB);                                                                      \
     break;
     LC_WAVQ_VARIANTS(X)
 #undef X
@@ -186,7 +188,7 @@ static int wav_zchunk(long tiles, int narr, int nout) {
   return pick_zchunk(tiles, nout, std::max(narr, 1), std::max(1, Op::W / 2), num_cu());
 }
 
-template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE, int ZC>
+template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE, int ZC, int KB>
 static hipError_t launch_wavq(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
   if (!wavq_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, 1, 1>(b, narr, x, y, a, a_y);
@@ -195,7 +197,7 @@ static hipError_t launch_wavq(const Box3& b, int narr, const R* x, R* y, R a, R 
   const int zch = (p.d2.nout + p.zchunk - 1) / p.zchunk;
   if (narr == 0) return hipSuccess;
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
-  wavq_fused<R, Op, TX, TY, QX, PD, WPE><<<grid, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(p);
+  wavq_fused<R, Op, TX, TY, QX, PD, WPE, KB><<<grid, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(p);
   return hipGetLastError();
 }
 
@@ -207,8 +209,8 @@ static hipError_t sepw3d_variant(int v, const Box3& b, int narr, const R* x, R* 
   case I: return launch_sepw3d<R, Op, TX, TY, QX, BZ, AD, WPE, ZC>(b, narr, x, y, a, a_y, st);
     LC_WAV_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, PD, WPE, ZC) \
-  case I: return launch_wavq<R, Op, TX, TY, QX, PD, WPE, ZC>(b, narr, x, y, a, a_y, st);
+#define X(I, TX, TY, QX, PD, WPE, ZC, KB) \
+  case I: return launch_wavq<R, Op, TX, TY, QX, PD, WPE, ZC, KB>(b, narr, x, y, a, a_y, st);
     LC_WAVQ_VARIANTS(X)
 #undef X
   }
@@ -272,7 +274,7 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
   case I: return wav_stage_build<R, Op, TX, TY, QX, BZ, AD, WPE>(it, count, s);
       LC_WAV_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, PD, WPE, ZC)                                      \
+#define X(I, TX, TY, QX, PD, WPE, ZC, KB)                                  \
   case I:                                                                  \
     for (int o = 0; o < count; ++o)                                        \
       if (!wavq_fits(it[o].b, sizeof(R))) return hipErrorInvalidValue;     \
@@ -291,9 +293,9 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
     break;
     LC_WAV_VARIANTS(X)
 #undef X
-#define X(I, TX, TY, QX, PD, WPE, ZC)                                                                     \
+#define X(I, TX, TY, QX, PD, WPE, ZC, KB)                                                                 \
   case I:                                                                                                 \
-    wavq_batch<R, Op, TX, TY, QX, PD, WPE><<<s->blocks, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(tab, s->first, \
+    wavq_batch<R, Op, TX, TY, QX, PD, WPE, KB><<<s->blocks, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(tab, s->first, \
                                                                                                 s->count); \
     break;
     LC_WAVQ_VARIANTS(X)

@@ -64,7 +64,7 @@ def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v, bc):
     assert np.abs(dot.cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
 
 
-@pytest.mark.parametrize('v', range(12))
+@pytest.mark.parametrize('v', range(16))
 @pytest.mark.parametrize('inverse', [False, True])
 def test_every_wavelet_variant_matches_oracle(lc, gpu, monkeypatch, v, inverse):
     torch = gpu

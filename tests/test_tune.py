@@ -19,8 +19,8 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 def test_variant_counts(lc):
     assert lc.variant_count(lc.FORM_SEP) == 73
     assert lc.variant_count(lc.FORM_SUM) == 57
-    assert lc.variant_count(lc.FORM_DWT) == 12
-    assert lc.variant_count(lc.FORM_IDWT) == 12
+    assert lc.variant_count(lc.FORM_DWT) == 16
+    assert lc.variant_count(lc.FORM_IDWT) == 16
 
 
 def test_default_kernel_names(lc):
