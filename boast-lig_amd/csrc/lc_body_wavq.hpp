@@ -32,6 +32,16 @@
 
 namespace lc {
 
+// the shapes this kernel takes: whole 16-byte pieces per x-task, whole waves
+// of x- and y-tasks (some shapes exist in one precision only)
+template <typename R, int TX, int TY, int QX>
+constexpr bool wavq_shape_ok() {
+  if (QX <= 0 || QX > TX || TX % QX != 0 || QX % Vec16<R>::N != 0) return false;
+  const int nr = 2 * (TY + 7), ars = 64 / igcd(64, TX / QX);
+  const int ar = (nr + ars - 1) / ars * ars;
+  return (ar * (TX / QX)) % 64 == 0 && (2 * TX * TY) % 64 == 0;
+}
+
 template <typename R, class Op, int TX, int TY, int QX>
 struct WavqGeo {
   static constexpr int W = Op::W, V = Vec16<R>::N;

@@ -96,9 +96,10 @@ __global__ __launch_bounds__((WavqGeo<R, Op, TX, TY, QX>::NT)) __attribute__((am
 #define LC_WAVQ_VARIANTS(X)                                                                                  \
   X(7, 16, 8, 4, 2, 1, -2, 0) X(8, 16, 8, 4, 2, 1, -3, 0) X(9, 32, 8, 4, 2, 1, -2, 0)                           \
   X(10, 16, 16, 4, 2, 1, -2, 0) X(11, 16, 8, 4, 4, 1, -2, 0) X(12, 16, 8, 4, 2, 4, -2, 2)                        \
-  X(13, 16, 16, 4, 2, 4, -2, 2) X(14, 16, 8, 4, 2, 4, -2, 1) X(15, 16, 8, 4, 2, 1, -2, 2)
+  X(13, 16, 16, 4, 2, 4, -2, 2) X(14, 16, 8, 4, 2, 4, -2, 1) X(15, 16, 8, 4, 2, 1, -2, 2)                         \
+  X(16, 8, 8, 2, 2, 1, -2, 0) X(17, 8, 8, 2, 2, 4, -2, 1)
 
-int wav_variant_count() { return 16; }
+int wav_variant_count() { return 18; }
 
 // output planes reached through 32-bit buffer records (the plane-at-a-time
 // variants store with buffer stores)
@@ -190,6 +191,9 @@ static int wav_zchunk(long tiles, int narr, int nout) {
 
 template <typename R, class Op, int TX, int TY, int QX, int PD, int WPE, int ZC, int KB>
 static hipError_t launch_wavq(const Box3& b, int narr, const R* x, R* y, R a, R a_y, hipStream_t st) {
+  if constexpr (!wavq_shape_ok<R, TX, TY, QX>()) {
+    return hipErrorInvalidValue;  // a shape of the other precision
+  } else {
   if (!wavq_fits(b, sizeof(R))) return hipErrorInvalidValue;
   FusedArgs<R> p = wav_args<R, Op, TX, TY, QX, 1, 1>(b, narr, x, y, a, a_y);
   const long tiles = (long)p.tiles_x * p.tiles_y;
@@ -199,6 +203,7 @@ static hipError_t launch_wavq(const Box3& b, int narr, const R* x, R* y, R a, R 
   dim3 grid((unsigned)tiles, (unsigned)zch, (unsigned)narr);
   wavq_fused<R, Op, TX, TY, QX, PD, WPE, KB><<<grid, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(p);
   return hipGetLastError();
+  }
 }
 
 template <typename R, class Op>
@@ -276,6 +281,7 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
 #undef X
 #define X(I, TX, TY, QX, PD, WPE, ZC, KB)                                  \
   case I:                                                                  \
+    if (!wavq_shape_ok<R, TX, TY, QX>()) return hipErrorInvalidValue;     \
     for (int o = 0; o < count; ++o)                                        \
       if (!wavq_fits(it[o].b, sizeof(R))) return hipErrorInvalidValue;     \
     return wav_stage_build<R, Op, TX, TY, QX, 1, 1, WPE>(it, count, s);
@@ -295,8 +301,11 @@ static hipError_t wav_stage_t(bool build, const BatchItem<R>* it, int count, Bat
 #undef X
 #define X(I, TX, TY, QX, PD, WPE, ZC, KB)                                                                 \
   case I:                                                                                                 \
-    wavq_batch<R, Op, TX, TY, QX, PD, WPE, KB><<<s->blocks, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(tab, s->first, \
-                                                                                                s->count); \
+    if constexpr (wavq_shape_ok<R, TX, TY, QX>())                                                          \
+      wavq_batch<R, Op, TX, TY, QX, PD, WPE, KB><<<s->blocks, WavqGeo<R, Op, TX, TY, QX>::NT, 0, st>>>(    \
+          tab, s->first, s->count);                                                                       \
+    else                                                                                                  \
+      return hipErrorInvalidValue;                                                                        \
     break;
     LC_WAVQ_VARIANTS(X)
 #undef X

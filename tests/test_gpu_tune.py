@@ -64,7 +64,7 @@ def test_every_fused_variant_matches_oracle(lc, gpu, monkeypatch, v, bc):
     assert np.abs(dot.cpu().numpy() - rdot).max() <= 1e-12 * np.abs(rdot).max()
 
 
-@pytest.mark.parametrize('v', range(16))
+@pytest.mark.parametrize('v', range(18))
 @pytest.mark.parametrize('inverse', [False, True])
 def test_every_wavelet_variant_matches_oracle(lc, gpu, monkeypatch, v, inverse):
     torch = gpu
@@ -131,10 +131,18 @@ def test_all_wavelet_variants_give_identical_bits(lc, gpu, monkeypatch, inverse)
         y0 = torch.from_numpy(rng.uniform(-1, 1, 8 * int(np.prod(ny)))).to('cuda', dt)
         for a, a_y in ((1.0, 0.0), (0.75, -0.5)):
             outs = []
+            prec = 8 if dt == torch.float64 else 4
             for v in range(lc.variant_count(lc.FORM_DWT)):
                 monkeypatch.setenv('LIBCONV_WAV3D_VARIANT', str(v))
                 y = y0.clone()
-                lc.wavelet_md(lc.WAV_SYM8, inverse, n, bc, nx, ny, 1, x, y, a=a, a_y=a_y)
+                try:
+                    lc.wavelet_md(lc.WAV_SYM8, inverse, n, bc, nx, ny, 1, x, y, a=a, a_y=a_y)
+                except lc.LibconvError:
+                    # plane-at-a-time shapes with x-tasks of part of a 16-byte
+                    # piece exist in FP64 only
+                    name = lc.kernel_name(prec, lc.FORM_IDWT if inverse else lc.FORM_DWT, lc.WAV_SYM8, n, bc, nx, ny)
+                    assert name.startswith('wavq_fused') and prec == 4, (v, name)
+                    continue
                 outs.append(y)
             torch.cuda.synchronize()
             for v, y in enumerate(outs):
@@ -169,7 +177,9 @@ def test_tune_md_records_fastest(lc, gpu):
     assert r['best_ms'] == min(r['times'])
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **shape) != ''
     w = lc.tune_md(lc.FORM_IDWT, lc.WAV_SYM8, repeat=2, prec=4, **shape)
-    assert 0 <= w['best'] < lc.variant_count(lc.FORM_IDWT) and np.isfinite(w['times']).all()
+    # FP32: every variant but the FP64-only plane-at-a-time shapes (16, 17)
+    assert 0 <= w['best'] < lc.variant_count(lc.FORM_IDWT) and np.isfinite(w['times'][:16]).all()
+    assert not np.isfinite(w['times'][16:]).any()
 
 
 def test_batch_tune_keeps_outputs(lc, gpu):

@@ -19,8 +19,8 @@ K256 = dict(n=[256] * 3, bc=-1, nx=[284] * 3, ny=[256] * 3)
 def test_variant_counts(lc):
     assert lc.variant_count(lc.FORM_SEP) == 73
     assert lc.variant_count(lc.FORM_SUM) == 57
-    assert lc.variant_count(lc.FORM_DWT) == 16
-    assert lc.variant_count(lc.FORM_IDWT) == 16
+    assert lc.variant_count(lc.FORM_DWT) == 18
+    assert lc.variant_count(lc.FORM_IDWT) == 18
 
 
 def test_default_kernel_names(lc):
@@ -31,7 +31,11 @@ def test_default_kernel_names(lc):
     assert lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
         'sepq_fused<double, lc::FMD, 32, 16, 4, 2, 2, 2>'
     assert lc.kernel_name(8, lc.FORM_IDWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).startswith(
-        'sepw3d_fused<double, lc::Idwt<')
+        'wavq_fused<double, lc::Idwt<')
+    assert lc.kernel_name(8, lc.FORM_IDWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).endswith(
+        ', 8, 8, 2, 2, 1, 0>')
+    assert lc.kernel_name(4, lc.FORM_DWT, lc.WAV_SYM8, [96] * 3, 0, [96] * 3, [96] * 3).endswith(
+        ', 16, 8, 4, 2, 1, 0>')
 
 
 def test_planes_beyond_buffer_records_use_pointer_kernels(lc):
