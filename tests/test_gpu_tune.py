@@ -111,6 +111,28 @@ def test_all_separable_variants_give_identical_bits(lc, gpu, monkeypatch):
             assert torch.equal(y, outs[0]), (str(dt), v)
 
 
+@pytest.mark.parametrize('v', range(18))
+def test_wavelet_variants_on_small_and_multi_array_shapes(lc, gpu, monkeypatch, v):
+    """Every wavelet variant on grids smaller than a tile and than the
+    filter (periodic dims of 2 and 3 steps wrap several times; a free dim of
+    2 steps), with two arrays per call, against the FP64 oracle."""
+    torch = gpu
+    rng = np.random.default_rng(1350 + v)
+    monkeypatch.setenv('LIBCONV_WAV3D_VARIANT', str(v))
+    for n, bc in (([3, 5, 2], [0, 0, 0]), ([2, 9, 3], [1, -1, 0]), ([11, 3, 2], [-1, 0, 1])):
+        nx = [k + 7 if b == -1 else k for k, b in zip(n, bc)]
+        ny = [k + 7 if b == 1 else k for k, b in zip(n, bc)]
+        narr = 2
+        for inverse in (False, True):
+            x = rng.uniform(-1, 1, 8 * int(np.prod(nx)) * narr)
+            ref = np.zeros(8 * int(np.prod(ny)) * narr)
+            (O.s1s0_md if inverse else O.s0s1_md)(O.SYM8, n, bc, nx, ny, narr, x, ref)
+            y = torch.zeros(ref.size, dtype=torch.float64, device='cuda')
+            lc.wavelet_md(lc.WAV_SYM8, inverse, n, bc, nx, ny, narr, torch.from_numpy(x).cuda(), y)
+            torch.cuda.synchronize()
+            assert rel(y.cpu().numpy(), ref) < TOL64, (n, bc, inverse)
+
+
 @pytest.mark.parametrize('inverse', [False, True])
 def test_all_wavelet_variants_give_identical_bits(lc, gpu, monkeypatch, inverse):
     """The step-at-a-time and plane-at-a-time wavelet kernels sum in the same
