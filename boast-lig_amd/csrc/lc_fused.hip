@@ -196,20 +196,22 @@ int select_variant(int prec, int form, int id, const Box3& b) {
   if (form == FORM_SUM || form == FORM_SEP) {
     v = env_variant(form == FORM_SUM ? "LIBCONV_SUM3D_VARIANT" : "LIBCONV_SEP3D_VARIANT", variant_count(form));
     if (v >= 0) return v;
-    // defaults: the fastest variants on MI355X (MF on output planes of
-    // 200^2 points and more: the 128-bit-LDS kernel, 32 x 16 tiles in FP64,
-    // 32 x 32 in FP32; on smaller planes, e.g. the 157^3 magic filter of the
-    // local Hamiltonian at 64^3 pairs, whose few tiles the z chunking splits
-    // into short chunks: the pipelined kernel; K: the two-plane kernel with
-    // at most two blocks per CU, in FP64 with the column planes two steps in
-    // flight); the plane-buffer forms (16 and up) take planes of at most
-    // kPlaneMax bytes
+    // defaults: the fastest variants on MI355X (MF in FP64: the 128-bit-LDS
+    // kernel on 32 x 16 tiles, 4-10% ahead of the others from the 128^3
+    // chain and the 157^3 magic filter of the local Hamiltonian at 64^3 pairs
+    // to the 284^3 chain; in FP32: the 128-bit-LDS kernel on 32 x 32 tiles on
+    // output planes of 200^2 points and more, the pipelined kernel with two
+    // outputs per task on smaller ones; K: the two-plane kernel with at most
+    // two blocks per CU, in FP64 with the column planes two steps in flight);
+    // the plane-buffer forms (16 and up) take planes of at most kPlaneMax
+    // bytes
     const bool fit = plane_fits(b, prec);
     if (form == FORM_SEP) {
       if (!fit) return 12;
       const int g = FMD::N - 1;  // outputs per dim: grow adds the filter length
       const long plane = (long)(b.bc[0] == GROW ? b.n[0] + g : b.n[0]) * (b.bc[1] == GROW ? b.n[1] + g : b.n[1]);
-      return plane >= 200L * 200 ? (prec == 4 ? 68 : 62) : 45;
+      if (prec == 8) return 62;
+      return plane >= 200L * 200 ? 68 : 41;
     }
     if (!fit) return prec == 4 ? 7 : 8;
     return prec == 4 ? 32 : 46;
@@ -261,6 +263,11 @@ hipError_t sep3d_fused_run(int id, const Box3& b, int narr, const R* x, R* y, R 
   int v = select_variant(sizeof(R), FORM_SEP, id, b);
   if (pot && v >= 5 && v <= 7)  // the wave-specialised forms have no potential step
     v = plane_fits(b, sizeof(R)) ? (sizeof(R) == 4 ? 28 : 33) : 12;
+  // with the potential step the 128-bit-LDS kernels take their general
+  // epilogue (stores inside the y-waves' branch, conservative wait counts):
+  // the pipelined kernel is faster there (the local Hamiltonian's 157^3
+  // magic filter: 0.203 against 0.226 ms for the whole chain)
+  if (pot && v >= 62 && v <= 72) v = sizeof(R) == 4 ? 41 : 45;
   return sep3d_fused_v<R>(v, id, b, narr, x, y, a, a_y, st, pot, rho);
 }
 
