@@ -309,7 +309,12 @@ __global__ __launch_bounds__(kPThreads) void grad_point_kernel(PGeom g, int segs
 // non-periodic z the 2m border planes are left to the one-point kernels, and
 // the march covers planes m .. n - m - 1.  Taps, order and operations equal
 // pderiv's, so both forms give identical bits.
-constexpr int kZTX = 32, kZTY = 8, kZNT = kZTX * kZTY;
+constexpr int kZTX = 32;
+// tile rows (every mode): 16 rows read half the y-halo rows per output that
+// 8 rows do (update_rhopol at 256^3, order 16: 0.207 -> 0.184 ms; div_u_i
+// with cc 0.285 -> 0.277 ms)
+template <int MODE>
+constexpr int pz_ty() { return 16; }
 
 // MODE 0: div_u_i, 1: div_u_i with cc, 2..5: the gradient family (GE_*).
 // Derivatives taken of input k (X, Y in LDS; Z in the register window).
@@ -320,13 +325,14 @@ struct PNeed {
   static constexpr bool TILE = X || Y || MODE == 1;  // cc also needs the centre values
 };
 
-template <typename R, int NORD>
+template <typename R, int NORD, int TY>
 struct PzGeo {
   static constexpr int M = NORD / 2, W = NORD + 1;
-  static constexpr int TWP = (kZTX + 2 * M) | 1, TH = kZTY + 2 * M;
+  static constexpr int NT = kZTX * TY;  // lanes: one output column each
+  static constexpr int TWP = (kZTX + 2 * M) | 1, TH = TY + 2 * M;
   static constexpr int TSZ = TH * TWP + 1;  // one tile plus a spare slot
-  static constexpr int nload(bool x, bool y) { return (kZTY + (y ? 2 * M : 0)) * (kZTX + (x ? 2 * M : 0)); }
-  static constexpr int nlane(bool x, bool y) { return (nload(x, y) + kZNT - 1) / kZNT; }
+  static constexpr int nload(bool x, bool y) { return (TY + (y ? 2 * M : 0)) * (kZTX + (x ? 2 * M : 0)); }
+  static constexpr int nlane(bool x, bool y) { return (nload(x, y) + NT - 1) / NT; }
 };
 
 // first index of a tile's LDS window along one dim: the centred halo, or on
@@ -342,7 +348,7 @@ __device__ __forceinline__ int pz_window(int o0, int t, int m, int n, bool per) 
 // the lane's entries of input k's tile: LDS slot and element offset in a plane
 template <typename R, int NORD, int MODE, int K>
 struct PzTile {
-  using G = PzGeo<R, NORD>;
+  using G = PzGeo<R, NORD, pz_ty<MODE>()>;
   using N = PNeed<MODE, K>;
   static constexpr int NL = N::TILE ? G::nlane(N::X, N::Y) : 0;
   int slot[NL > 0 ? NL : 1], boff[NL > 0 ? NL : 1];
@@ -353,7 +359,7 @@ struct PzTile {
     const int rx0 = N::X ? wx0 : X0, ry0 = N::Y ? wy0 : Y0;  // region origin
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
-      const int e = t + j * kZNT;
+      const int e = t + j * G::NT;
       const int gx = rx0 + e % NC, gy = ry0 + e / NC;
       const int sx = g.per0 ? pmod(gx, g.n0) : gx, sy = g.per1 ? pmod(gy, g.n1) : gy;
       // on a non-periodic dim only [0, n) is ever read
@@ -373,31 +379,35 @@ struct PzTile {
   }
 };
 
-// waves per SIMD asked of the compiler: 4 (<= 128 VGPRs), 2 for the FP64 form
-// with cc, whose three tiles and six stencils would spill at 128 or 168
+// waves per SIMD asked of the compiler: 4 (<= 128 VGPRs; the FP64 form with
+// cc at order 16 spills three values at that budget, on its 16-row tiles)
 template <typename R, int MODE>
-constexpr int pz_waves() { return MODE == 1 && sizeof(R) == 8 ? 2 : 4; }
+constexpr int pz_waves() { return 4; }
+
+template <int MODE>
+constexpr int pz_nt() { return kZTX * pz_ty<MODE>(); }  // lanes of a z-march block
 
 template <typename R, int NORD, int MODE>
-__global__ __launch_bounds__(kZNT) __attribute__((amdgpu_waves_per_eu(pz_waves<R, MODE>()))) void poisson_zm_kernel(
+__global__ __launch_bounds__(pz_nt<MODE>()) __attribute__((amdgpu_waves_per_eu(pz_waves<R, MODE>()))) void poisson_zm_kernel(
     PGeom g, int tiles_x, int tiles, int zlo, int zhi, int zchunk, const R* __restrict__ u, R* __restrict__ o0,
     R* __restrict__ o1, GradEpi<R> ep, R rh0, R rh1, R rh2) {
-  using G = PzGeo<R, NORD>;
+  constexpr int TY = pz_ty<MODE>(), NT = pz_nt<MODE>();
+  using G = PzGeo<R, NORD, TY>;
   constexpr int M = G::M, W = G::W, TWP = G::TWP, TSZ = G::TSZ;
   constexpr int NI = MODE >= GE_GRAD ? 1 : 3;
   __shared__ __align__(16) R lds[NI * TSZ];
   __shared__ R rows[W * W];  // every coefficient row (border lanes index it)
-  for (int k = threadIdx.x; k < W * W; k += kZNT) rows[k] = prow<R, NORD>(k);
+  for (int k = threadIdx.x; k < W * W; k += NT) rows[k] = prow<R, NORD>(k);
   const int lin = xcd_linear();
   const int tile = lin % tiles, zc = lin / tiles;
-  const int X0 = (tile % tiles_x) * kZTX, Y0 = (tile / tiles_x) * kZTY;
+  const int X0 = (tile % tiles_x) * kZTX, Y0 = (tile / tiles_x) * TY;
   const int zb = zlo + zc * zchunk, ze = min(zb + zchunk, zhi);
   const int t = threadIdx.x, tx = t % kZTX, ty = t / kZTX;
   const int i0 = X0 + tx, i1 = Y0 + ty;
   const bool out_ok = i0 < g.n0 && i1 < g.n1;
   const long plane = (long)g.n0 * g.n1;
   const R* uz = u + (NI - 1) * g.np;  // the input taken along z
-  const int wx0 = pz_window(X0, kZTX, M, g.n0, g.per0), wy0 = pz_window(Y0, kZTY, M, g.n1, g.per1);
+  const int wx0 = pz_window(X0, kZTX, M, g.n0, g.per0), wy0 = pz_window(Y0, TY, M, g.n1, g.per1);
 
   // the lane's stencils in the LDS window: first tap (row / column) and
   // coefficient row; border lanes of non-periodic dims take the shifted rows
@@ -519,8 +529,8 @@ __global__ __launch_bounds__(kZNT) __attribute__((amdgpu_waves_per_eu(pz_waves<R
     if (z + 3 < ze) step(z + 3, 1);
   }
   if constexpr (MODE == GE_RHOPOL) {
-    __shared__ double red[kZNT / 64];
-    const double s = pblock_sum<kZNT>(sq, red);
+    __shared__ double red[NT / 64];
+    const double s = pblock_sum<NT>(sq, red);
     if (threadIdx.x == 0) ep.part[blockIdx.x] = s;
   }
 }
@@ -579,7 +589,7 @@ struct PzLaunch {
   int tiles_x, tiles, zlo, zhi, zchunk;
   unsigned blocks;
 };
-static bool pz_launch(const PGeom& g, int nord, int resident, PzLaunch& L) {
+static bool pz_launch(const PGeom& g, int nord, int ty, int resident, PzLaunch& L) {
   if (getenv("LIBCONV_POISSON_POINT")) return false;
   if ((long)g.n0 * g.n1 * 8 > kPlaneMax) return false;
   const int m = nord / 2;
@@ -587,7 +597,7 @@ static bool pz_launch(const PGeom& g, int nord, int resident, PzLaunch& L) {
   L.zhi = g.per2 ? g.n2 : g.n2 - m;
   const int nz = L.zhi - L.zlo;  // >= 1: n >= nord + 1
   L.tiles_x = (g.n0 + kZTX - 1) / kZTX;
-  const long tiles = (long)L.tiles_x * ((g.n1 + kZTY - 1) / kZTY);
+  const long tiles = (long)L.tiles_x * ((g.n1 + ty - 1) / ty);
   long chunks = (long)std::max(1, resident) * num_cu() / tiles;
   if (chunks * tiles < num_cu()) chunks = (num_cu() + tiles - 1) / tiles;
   chunks = std::max(1L, std::min<long>(chunks, nz / (2 * (nord + 1))));
@@ -600,12 +610,12 @@ static bool pz_launch(const PGeom& g, int nord, int resident, PzLaunch& L) {
 }
 
 // blocks of one z-march kernel that a CU holds (registers, LDS), cached
-template <auto KER>
+template <auto KER, int NT>
 static int pz_resident() {
   static int nb = 0;
   if (!nb) {
     int v = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, KER, kZNT, 0) != hipSuccess || v < 1) v = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, KER, NT, 0) != hipSuccess || v < 1) v = 1;
     nb = v;
   }
   return nb;
@@ -616,10 +626,10 @@ template <typename R, int NO, int MODE>
 static hipError_t pz_run(const PGeom& g, const R* u, R* o0, R* o1, R r0, R r1, R r2, hipStream_t st,
                          bool* done) {
   PzLaunch L;
-  *done = pz_launch(g, NO, pz_resident<poisson_zm_kernel<R, NO, MODE>>(), L);
+  *done = pz_launch(g, NO, pz_ty<MODE>(), pz_resident<poisson_zm_kernel<R, NO, MODE>, pz_nt<MODE>()>(), L);
   if (!*done) return hipSuccess;
-  poisson_zm_kernel<R, NO, MODE><<<L.blocks, kZNT, 0, st>>>(g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u, o0,
-                                                              o1, GradEpi<R>{}, r0, r1, r2);
+  poisson_zm_kernel<R, NO, MODE><<<L.blocks, pz_nt<MODE>(), 0, st>>>(g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk,
+                                                                     u, o0, o1, GradEpi<R>{}, r0, r1, r2);
   return hipGetLastError();
 }
 
@@ -677,13 +687,13 @@ static long grad_launch(const PGeom& g, const R* u, const GradEpi<R>& ep, R r0, 
     off += nb;
   };
   PzLaunch L;
-  if (!pz_launch(g, NO, pz_resident<poisson_zm_kernel<R, NO, MODE>>(), L)) {
+  if (!pz_launch(g, NO, pz_ty<MODE>(), pz_resident<poisson_zm_kernel<R, NO, MODE>, pz_nt<MODE>()>(), L)) {
     point(g);
     return off;
   }
   if (!dry)
-    poisson_zm_kernel<R, NO, MODE><<<L.blocks, kZNT, 0, st>>>(g, L.tiles_x, L.tiles, L.zlo, L.zhi, L.zchunk, u,
-                                                                nullptr, nullptr, ep, r0, r1, r2);
+    poisson_zm_kernel<R, NO, MODE><<<L.blocks, pz_nt<MODE>(), 0, st>>>(g, L.tiles_x, L.tiles, L.zlo, L.zhi,
+                                                                       L.zchunk, u, nullptr, nullptr, ep, r0, r1, r2);
   off += L.blocks;
   pz_border_planes(g, NO, point);
   return off;
