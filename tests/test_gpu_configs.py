@@ -236,7 +236,7 @@ def test_config5_batch_512(lc, gpu, dtype_name):
         rd = O.s0s0_1ds(O.D2, n_k, [bc] * 3, y1s, y2s, 1, a, 0.0, 0.0, r1, r2)
         e1 = errs(mf[k]['y'].double().cpu().numpy(), r1)
         # the Kinetic step of the GPU chain reads the GPU's y1: compare it with
-        # the oracle on that same input, so that each step is judged alone
+        # the oracle on that same input, so that each step is checked alone
         y1g = kin[k]['x'].double().cpu().numpy()
         r2g = np.zeros(r2.size)
         rdg = O.s0s0_1ds(O.D2, n_k, [bc] * 3, y1s, y2s, 1, a, 0.0, 0.0, y1g, r2g)
