@@ -704,6 +704,10 @@ int run_dielectric(int op, int geocode, const int* n, const R* u, const R* in1, 
   if (!h) return fail(LC_ERR_ARG, "%s: null hgrids", who);
   if (!u || !out || (op != PD_SQUARE && !in1) || (op == PD_RHOPOL && !rhores2))
     return fail(LC_ERR_ARG, "%s: null array", who);
+  // the stream form writes rhores2 from the device: a host address there
+  // would fault the kernel, so it is refused here
+  if (device_api && op == PD_RHOPOL && !is_device_ptr(rhores2))
+    return fail(LC_ERR_ARG, "%s: rhores2 is not device memory", who);
   const size_t np = (size_t)n[0] * n[1] * n[2];
   Arg<R> au, a1, ao;
   double* r2 = rhores2;

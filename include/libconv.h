@@ -371,8 +371,9 @@ int lc_nabla_u_epsilon(int prec, int geocode, const int *n, const void *u, void 
 /* per point res = ((e_1 d_x u + e_2 d_y u) + e_3 d_z u) / (4 pi), with
  * (e_1, e_2, e_3) = dlogeps(:, i0, i1, i2) ((3, n0, n1, n2), the component
  * fastest); res = eta (res - rhopol); rhopol = res + rhopol (in place);
- * *rhores2_dev (device FP64) = the sum of res^2 over the grid, summed in FP64
- * in a fixed order (the same bits on every call)                           */
+ * *rhores2_dev (device FP64; a host address is refused) = the sum of res^2
+ * over the grid, summed in FP64 in a fixed order (the same bits on every
+ * call)                                                                    */
 int lc_update_rhopol(int prec, int geocode, const int *n, const void *u, int nord, const double *h,
                      double eta, const void *dlogeps, void *rhopol, double *rhores2_dev, void *stream);
 /* The generated procedure names (C flavour, FP64, host or device arrays,

@@ -159,7 +159,7 @@ def test_noalloc_entry_point_refuses_oversized_partials_before_the_device(lib):
     assert (buf == 0).all()
 
 
-@pytest.mark.parametrize('case', ['nord', 'small_n', 'null_h', 'null_rhores2', 'prec'])
+@pytest.mark.parametrize('case', ['nord', 'small_n', 'null_h', 'null_rhores2', 'host_rhores2', 'prec'])
 def test_poisson_dielectric_entry_points_validate_before_the_device(lib, case):
     """lc_nabla_u_square, lc_nabla_u_epsilon and lc_update_rhopol refuse a
     bad order, a grid no larger than the stencil, a missing hgrids or output
@@ -180,10 +180,13 @@ def test_poisson_dielectric_entry_points_validate_before_the_device(lib, case):
         hp = None
     elif case == 'null_rhores2':
         r2 = None
+    elif case == 'host_rhores2':  # every other argument valid: refused for the host address alone
+        host = (ctypes.c_double * 1)()
+        r2 = ctypes.cast(host, ctypes.POINTER(ctypes.c_double))
     elif case == 'prec':
         prec = 2
     calls = [lambda: L.lc_update_rhopol(prec, 1, n, dummy, nord, hp, ctypes.c_double(0.5), dummy, dummy, r2, None)]
-    if case != 'null_rhores2':
+    if case not in ('null_rhores2', 'host_rhores2'):
         calls += [lambda: L.lc_nabla_u_square(prec, 1, n, dummy, dummy, nord, hp, None),
                   lambda: L.lc_nabla_u_epsilon(prec, 1, n, dummy, dummy, nord, hp, dummy, None)]
     for call in calls:
