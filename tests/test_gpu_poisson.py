@@ -299,6 +299,27 @@ def test_dielectric_z_march_equals_point_kernels(lc, gpu, monkeypatch, nord, geo
     assert abs(zm[3] - pt[3]) <= 1e-12 * zm[3]
 
 
+@pytest.mark.parametrize('nord', [2, 8, 16])
+@pytest.mark.parametrize('geocode', [1, 2, 3])
+def test_dielectric_smallest_grids(lc, gpu, nord, geocode):
+    """The three operators on grids of nord + 1 points along every dim (the
+    smallest div_u_i_ref accepts: on a free dim every point is a border
+    point, and the z march covers a single plane) and on a mixed one."""
+    torch = gpu
+    for n in ([nord + 1] * 3, [nord + 1, nord + 2, nord + 3]):
+        rng = np.random.default_rng(4400 + 10 * nord + geocode + n[2])
+        h = [0.2, 0.3, 0.4]
+        u, eps, dl, rp = _dielectric_inputs(rng, n, False)
+        np_ = int(np.prod(n))
+        sq, de, rpo = np.zeros(np_), np.zeros(3 * np_), rp.copy()
+        O.nabla_u_square(geocode, n, u, sq, nord, h)
+        O.nabla_u_epsilon(geocode, n, u, de, nord, h, eps)
+        r2 = O.update_rhopol(geocode, n, u, nord, h, 0.9, dl, rpo)
+        gsq, gde, grp, gr2 = _run_dielectric(lc, torch, geocode, n, nord, h, 0.9, u, eps, dl, rp, torch.float64)
+        assert rel(gsq, sq) < TOL64 and rel(gde, de) < TOL64 and rel(grp, rpo) < TOL64, n
+        assert abs(gr2 - r2) <= 1e-12 * r2, n
+
+
 def test_dielectric_large_grid(lc, gpu):
     """A 200 x 180 x 160 grid (5.8 M points, many tiles and z chunks) in FP64
     against the oracle, nord 16, x and z periodic, y free."""
