@@ -657,6 +657,10 @@ def test_poisson_dielectric_identities(geo):
         sq = np.zeros(np_)
         O.nabla_u_square(geo, n, u, sq, nord, h)
         assert np.array_equal(sq, (gx * gx + gy * gy) + gz * gz)
+        # nabla_u_and_square: the gradient of nabla_u and the square of nabla_u_square
+        g2, s2 = np.zeros(3 * np_), np.zeros(np_)
+        O.nabla_u_and_square(geo, n, u, g2, s2, nord, h)
+        assert np.array_equal(g2, gr) and np.array_equal(s2, sq)
         eps = rng.uniform(0.5, 2, np_)
         de = np.zeros(3 * np_)
         O.nabla_u_epsilon(geo, n, u, de, nord, h, eps)
