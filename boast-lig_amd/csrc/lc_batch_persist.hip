@@ -71,8 +71,9 @@ using PSumGeo = Sum2Geo<R, FD2, 16, 16, 4, 4, 2>;
 template <typename R, class F, bool ME>
 __device__ __forceinline__ void p_sep(const FusedArgs<R>& p, int bx, int by, int zc, unsigned char* lds) {
   using S = PShape<R>;
-  // streaming stores: little for the publishing fence to write back
-  sepq_body<R, F, S::MX, S::MY, S::MQX, S::MQY, 2, ME, true>(p, bx, by, zc, 0, lds);
+  // plain stores (streaming ones measured no faster for the publishing
+  // fence's write-back)
+  sepq_body<R, F, S::MX, S::MY, S::MQX, S::MQY, 2, ME, false>(p, bx, by, zc, 0, lds);
 }
 template <typename R, bool ME>
 __device__ __forceinline__ void p_sum(const FusedArgs<R>& p, int bx, int by, int zc, long bid, unsigned char* lds) {
@@ -135,7 +136,9 @@ __global__ __launch_bounds__(kPThreads) __attribute__((amdgpu_waves_per_eu(2))) 
           }
         }
       }
-      __threadfence();  // acquire: the dependencies' outputs
+      // acquire only (an L2 invalidate): a full fence would also write
+      // back this XCD's L2
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     const FusedArgs<R> p = tab[it];
@@ -156,14 +159,15 @@ __global__ __launch_bounds__(kPThreads) __attribute__((amdgpu_waves_per_eu(2))) 
       case PB_DWT: p_wav<R, Dwt<Sym8Dec>>(p, bx, by, zc, lds); break;
       case PB_IDWT: p_wav<R, Idwt<Sym8Rec>>(p, bx, by, zc, lds); break;
     }
-    // publish: this wave's stores complete, then the block, then one fence.
-    // The fence writes back the XCD's L2, so units nobody waits for skip it
-    // (their stores are complete when the kernel ends).
+    // publish: this wave's stores complete, then the block, then one release
+    // fence (an L2 write-back; a full fence would also invalidate the XCD's
+    // L2 under every other block running there).  Units nobody waits for
+    // skip it (their stores are complete when the kernel ends).
     if (!pi.publish) continue;  // uniform
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (t == 0) {
-      __threadfence();  // release
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       atomicAdd(&done[it], 1);
     }
   }
