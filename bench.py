@@ -658,6 +658,8 @@ def main():
                      'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': measured_traffic(kname),
                      'kernel': kname or dom,
                      'avg_ms': round(dom_t, 5), 'algorithmic_bytes': dom_b,
+                     'timed': ('HIP events around the operator call on its stream: the kernel, plus the '
+                               '~5 us energy reduction kernel (sum_partials) for the Kinetic operator'),
                      'copy_ceiling_GBs': ceiling, 'frac_of_copy': round(achieved / ceiling, 4)},
         'chain': {'GB_s': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9, 1),
                   'frac_hbm': round(chain_bytes / (r64['ms'] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
