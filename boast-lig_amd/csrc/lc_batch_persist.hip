@@ -6,7 +6,8 @@
 // lists its items by dependency depth, so that a unit only ever waits for
 // units of a lower depth.  A persistent grid of 8 k resident blocks (block b
 // serves queue b % 8, the XCD the dispatcher places it on) pops units with
-// an atomic counter until its queue is empty.
+// an atomic counter until its queue is empty, then serves the other queues
+// in turn until all are empty.
 //
 // Dependencies come from memory: item j depends on an earlier item i when
 // i writes what j reads or writes, or j writes what i reads.  A unit of j
@@ -21,8 +22,10 @@
 // item's per-block partials in a fixed order.
 //
 // Progress: all 8 k blocks are resident (k from the occupancy query), every
-// queue is ordered by depth, and depth-0 units never wait; by induction on
-// depth every unit completes.
+// queue is ordered by depth and popped in that order (by its own blocks, and
+// by others only after that), and depth-0 units never wait; by induction on
+// depth every unit completes: a popped unit waits only for units of lower
+// depth, and a queue's own blocks stay on it until it is empty.
 #include <stdio.h>
 
 #include <algorithm>
@@ -112,14 +115,20 @@ __global__ __launch_bounds__(kPThreads) __attribute__((amdgpu_waves_per_eu(2))) 
   __shared__ __align__(16) unsigned char lds[persist_lds_bytes<R>()];
   __shared__ int s_pos;
   const int t = threadIdx.x;
-  const int queue = blockIdx.x & 7;
-  const int qbeg = qoff[queue], qend = qoff[queue + 1];
+  // the block's own queue first; once that is empty, the other queues in
+  // turn (in their order), so that no XCD idles while another has units
+  // left; the block leaves when it has found all 8 empty
+  int queue = blockIdx.x & 7, empty = 0;
   for (;;) {
     __syncthreads();  // the previous unit is done with LDS and s_pos
     if (t == 0) s_pos = atomicAdd(&st->next[queue], 1);
     __syncthreads();
-    const int pos = __builtin_amdgcn_readfirstlane(qbeg + s_pos);
-    if (pos >= qend) break;  // uniform: every wave of the block leaves
+    const int pos = __builtin_amdgcn_readfirstlane(qoff[queue] + s_pos);
+    if (pos >= qoff[queue + 1]) {  // uniform
+      if (++empty == 8) break;     // every wave of the block leaves
+      queue = (queue + 1) & 7;
+      continue;
+    }
     const int2 wk = work[pos];
     const int it = __builtin_amdgcn_readfirstlane(wk.x), local = __builtin_amdgcn_readfirstlane(wk.y);
     const PItem pi = items[it];
