@@ -312,6 +312,36 @@ def test_persistent_equals_stage_launches(lc, gpu):
 
 
 @pytest.mark.parametrize('dtype_name', ['float64', 'float32'])
+def test_persistent_unbalanced_queues(lc, gpu, dtype_name):
+    """One large orbital among tiny ones: most queues run dry early and their
+    blocks take the large items' units from the other queues, Kinetic units
+    there waiting for magic-filter units run on other XCDs.  Outputs equal
+    the stage launches bit for bit, over replays."""
+    torch = gpu
+    dtype = getattr(torch, dtype_name)
+    rng = np.random.default_rng(1550)
+    orbs = [([88, 80, 72], -1)] + [([6, 7, 6], int(rng.choice([0, 1, -1]))) for _ in range(15)]
+    items, bufs = _build(torch, orbs, rng, dtype)
+    plan = lc.Batch(items, prec=8 if dtype_name == 'float64' else 4)
+    plan.run()
+    torch.cuda.synchronize()
+    ref = [[b.clone() for b in (y1, y2, dot, wy)] for (_, y1, y2, dot, _, wy) in bufs]
+    for rep in range(3):
+        for (_, y1, y2, dot, _, wy) in bufs:
+            for b in (y1, y2, dot, wy):
+                b.fill_(3.0 + rep)
+        plan.run_persistent()
+        torch.cuda.synchronize()
+        assert plan.persistent_info()['status'] == 0
+        for k, (_, y1, y2, dot, _, wy) in enumerate(bufs):
+            assert torch.equal(ref[k][0], y1), (rep, k)
+            assert torch.equal(ref[k][1], y2), (rep, k)
+            assert torch.equal(ref[k][3], wy), (rep, k)
+            d_p, d_s = dot.cpu().numpy(), ref[k][2].cpu().numpy()
+            assert np.abs(d_p - d_s).max() <= 1e-13 * np.abs(d_s).max(), (rep, k)
+
+
+@pytest.mark.parametrize('dtype_name', ['float64', 'float32'])
 def test_persistent_graph_replays(lc, gpu, dtype_name):
     torch = gpu
     dtype = getattr(torch, dtype_name)
