@@ -171,10 +171,14 @@ __device__ __forceinline__ void sum3d2_body(const FusedArgs<R>& p, int bx, int b
         for (int j = 0; j < NHL; ++j)
           T[q * TH * TW + hslot[j]] = hreg[q * NHL + j];
       // prefetch: halo planes one step ahead, column planes PW steps ahead
-      // (after the last steps: planes nobody uses)
+      // (after the last steps: planes nobody uses).  The wave issues them at
+      // a raised priority, so that they enter the memory pipeline ahead of
+      // the other waves' filter work on the SIMD (FP64 kernel -1.5%)
+      __builtin_amdgcn_s_setprio(1);
       load_halo(hreg);
 #pragma unroll
       for (int q = 0; q < QZ; ++q) nxt[u][q] = col_next();
+      __builtin_amdgcn_s_setprio(0);
       __syncthreads();
       // ---- one task per lane: x-outputs of a row or y-outputs of a column
       if (t < G::NXT) {
