@@ -202,7 +202,8 @@ int select_variant(int prec, int form, int id, const Box3& b) {
     // to the 284^3 chain; in FP32: the 128-bit-LDS kernel on 32 x 32 tiles on
     // output planes of 200^2 points and more, the pipelined kernel with two
     // outputs per task on smaller ones; K: the two-plane kernel with at most
-    // two blocks per CU, in FP64 with the column planes two steps in flight);
+    // two blocks per CU and two planes per step, in FP64 with the column
+    // planes two steps in flight);
     // the plane-buffer forms (16 and up) take planes of at most kPlaneMax
     // bytes
     const bool fit = plane_fits(b, prec);
@@ -214,7 +215,7 @@ int select_variant(int prec, int form, int id, const Box3& b) {
       return plane >= 200L * 200 ? 68 : 41;
     }
     if (!fit) return prec == 4 ? 7 : 8;
-    return prec == 4 ? 32 : 46;
+    return prec == 4 ? 31 : 46;
   }
   v = env_variant("LIBCONV_WAV3D_VARIANT", wav_variant_count());
   if (v >= 0) return v;

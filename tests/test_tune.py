@@ -25,7 +25,7 @@ def test_variant_counts(lc):
 
 def test_default_kernel_names(lc):
     assert lc.kernel_name(8, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2w_fused<double, lc::FD2, 32, 16, 8, 8, 2, 4, 0, 2>'
-    assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<float, lc::FD2, 32, 16, 8, 8, 4, 2, 8>'
+    assert lc.kernel_name(4, lc.FORM_SUM, lc.FIL_D2, **K256) == 'sum3d2_fused<float, lc::FD2, 32, 16, 8, 8, 2, 2, 0>'
     assert lc.kernel_name(4, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
         'sepq_fused<float, lc::FMD, 32, 32, 8, 4, 2, 2>'
     assert lc.kernel_name(8, lc.FORM_SEP, lc.FIL_MD, [284] * 3, -1, [299] * 3, [284] * 3) == \
